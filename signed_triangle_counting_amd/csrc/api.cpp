@@ -1,0 +1,412 @@
+// C ABI of the signed triangle counter (include/stc.h): context, model and
+// tune-table loading, and the host orchestration of upload -> CSR build ->
+// binning -> counting.  Replaces the reference's operator entry
+// gunrock::signed_tc::run (Signed_TC/GPU Implementations/GlobalMEM/
+// signed_tc.hxx:527-542) and the main()s of the OpenMP programs
+// (Signed_TC/Multicore Implementations/tc_merge_sign_enu.cpp:46-154).
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/stc.h"
+#include "internal.h"
+
+using namespace stc;
+
+struct stc_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int num_cus = 256;
+  std::string err;
+  int model_status = STC_OK;
+  DispatchParams hp{};
+  DispatchParams *dp = nullptr;
+  BuildBuffers bb;
+  CountBuffers cb;
+  BuildResult br;
+  DevCsr g;
+  bool prepared = false;
+  stc_stats stats{};
+  stc_classes classes{};
+  bool have_classes = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf batch_counts, batch_vbase;
+};
+
+namespace {
+
+int fail(stc_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(stc_ctx *c, hipError_t e, const char *where) {
+  (void)hipGetLastError();
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return fail(c, STC_E_OOM, m);
+  return fail(c, STC_E_HIP, m);
+}
+
+void default_tune(TuneParams &t) {
+  for (int i = 0; i < kBucketSide; i++) t.merge_wave_min_log2[i] = 5;
+  t.hash_max_log2 = 11;
+  t.bitmap_span_log2 = 17;
+  t.hash_block = 256;
+  t.fixed_skew_log2 = 2;
+  t.fixed_hub_log2 = 9;
+}
+
+std::string lib_dir() {
+  Dl_info info;
+  if (dladdr((void *)&default_tune, &info) && info.dli_fname) {
+    std::string p(info.dli_fname);
+    size_t s = p.find_last_of('/');
+    if (s != std::string::npos) return p.substr(0, s);
+  }
+  return ".";
+}
+
+uint64_t fnv1a(const std::string &s) {
+  uint64_t h = 1469598103934665603ull;
+  for (unsigned char ch : s) {
+    h ^= ch;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// Tune table: "stc-tune v1" then "key values..." lines.
+bool load_tune(const std::string &path, TuneParams &t) {
+  std::ifstream f(path);
+  if (!f) return false;
+  std::string line;
+  if (!std::getline(f, line) || line.rfind("stc-tune v1", 0) != 0) return false;
+  TuneParams x = t;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string k;
+    if (!(is >> k) || k[0] == '#') continue;
+    if (k == "merge_wave_min_log2") {
+      for (int i = 0; i < kBucketSide; i++)
+        if (!(is >> x.merge_wave_min_log2[i])) return false;
+    } else if (k == "hash_max_log2") {
+      if (!(is >> x.hash_max_log2) || x.hash_max_log2 < 4 || x.hash_max_log2 > 13) return false;
+    } else if (k == "bitmap_span_log2") {
+      if (!(is >> x.bitmap_span_log2) || x.bitmap_span_log2 < 8 || x.bitmap_span_log2 > 18)
+        return false;
+    } else if (k == "fixed_skew_log2") {
+      if (!(is >> x.fixed_skew_log2)) return false;
+    } else if (k == "fixed_hub_log2") {
+      if (!(is >> x.fixed_hub_log2)) return false;
+    }
+  }
+  t = x;
+  return true;
+}
+
+// Dispatcher model: "stc-dispatch-mlp v1", dims line, arrays by name, then
+// "check <fnv1a-64 of every preceding line>".  Any mismatch rejects the file.
+bool load_model(const std::string &path, MlpModel &m, std::string &why) {
+  std::ifstream f(path);
+  if (!f) {
+    why = "cannot open " + path;
+    return false;
+  }
+  std::string all, line;
+  std::vector<std::string> lines;
+  while (std::getline(f, line)) lines.push_back(line);
+  if (lines.size() < 3 || lines[0] != "stc-dispatch-mlp v1") {
+    why = "bad header";
+    return false;
+  }
+  std::string body;
+  size_t i = 0;
+  for (; i < lines.size(); i++) {
+    if (lines[i].rfind("check ", 0) == 0) break;
+    body += lines[i];
+    body += '\n';
+  }
+  if (i == lines.size()) {
+    why = "missing check line";
+    return false;
+  }
+  unsigned long long want = std::strtoull(lines[i].c_str() + 6, nullptr, 16);
+  if (fnv1a(body) != want) {
+    why = "checksum mismatch";
+    return false;
+  }
+  MlpModel x{};
+  auto read_arr = [&](const std::string &name, float *dst, int cnt) -> bool {
+    for (auto &l : lines) {
+      std::istringstream is(l);
+      std::string k;
+      is >> k;
+      if (k != name) continue;
+      for (int j = 0; j < cnt; j++)
+        if (!(is >> dst[j]) || !std::isfinite(dst[j])) return false;
+      return true;
+    }
+    return false;
+  };
+  bool ok = true;
+  {
+    std::istringstream is(lines[1]);
+    std::string k;
+    int a = 0, b = 0, c = 0;
+    is >> k >> a >> b >> c;
+    ok = k == "dims" && a == kMlpIn && b == kMlpHidden && c == kMlpOut;
+  }
+  ok = ok && read_arr("mean", x.mean, kMlpIn) && read_arr("inv_std", x.inv_std, kMlpIn) &&
+       read_arr("w1", &x.w1[0][0], kMlpHidden * kMlpIn) && read_arr("b1", x.b1, kMlpHidden) &&
+       read_arr("w2", &x.w2[0][0], kMlpOut * kMlpHidden) && read_arr("b2", x.b2, kMlpOut);
+  if (!ok) {
+    why = "bad dims or arrays";
+    return false;
+  }
+  m = x;
+  return true;
+}
+
+float median(std::vector<float> v) {
+  if (v.empty()) return 0.f;
+  std::sort(v.begin(), v.end());
+  size_t k = v.size();
+  return (k & 1) ? v[k / 2] : 0.5f * (v[k / 2 - 1] + v[k / 2]);
+}
+
+int check_graph(stc_ctx *c, const stc_edges *g) {
+  if (!g) return fail(c, STC_E_ARG, "null graph");
+  if (g->m && (!g->u || !g->v || !g->sign)) return fail(c, STC_E_ARG, "null edge arrays");
+  if (g->n > (1u << 31)) return fail(c, STC_E_TOO_LARGE, "n must be <= 2^31");
+  if (g->m >= (1ull << 32)) return fail(c, STC_E_TOO_LARGE, "at most 2^32-1 lines");
+  return STC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stc_create(int device, const char *model_dir, stc_ctx **out) {
+  if (!out) return STC_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return STC_E_NODEV;
+  }
+  if (device < 0 || device >= ndev) return STC_E_NODEV;
+  auto *c = new stc_ctx();
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->num_cus = prop.multiProcessorCount;
+    e = hipStreamCreate(&c->st);
+  }
+  for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+  if (e == hipSuccess) e = hipMalloc(&c->dp, sizeof(DispatchParams));
+  if (e != hipSuccess) {
+    int rc = hip_fail(c, e, "stc_create");
+    stc_destroy(c);
+    return rc;
+  }
+  default_tune(c->hp.tune);
+  std::string dir = model_dir ? std::string(model_dir) : lib_dir() + "/models";
+  load_tune(dir + "/tune.txt", c->hp.tune);
+  std::string why;
+  if (load_model(dir + "/dispatch_mlp.txt", c->hp.model, why)) {
+    c->hp.have_model = 1;
+    c->model_status = STC_OK;
+  } else {
+    c->hp.have_model = 0;
+    c->model_status = STC_E_MODEL;
+    c->err = "dispatcher model unavailable (" + why + "); fixed rule in use";
+  }
+  *out = c;
+  return STC_OK;
+}
+
+void stc_destroy(stc_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->dp) (void)hipFree(c->dp);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char *stc_last_error(const stc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+int stc_model_status(const stc_ctx *c) { return c ? c->model_status : STC_E_ARG; }
+
+int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
+  if (!c) return STC_E_ARG;
+  int rc = check_graph(c, g);
+  if (rc) return rc;
+  (void)hipSetDevice(c->device);
+  c->prepared = false;
+  const uint32_t *du = g->u, *dv = g->v;
+  const int8_t *ds = g->sign;
+  hipError_t e = hipSuccess;
+  float ms_up = 0.f;
+  if (!g_is_device) {
+    const uint64_t mm = g->m ? g->m : 1;
+    if ((e = c->bb.in_u.ensure(mm * 4)) != hipSuccess ||
+        (e = c->bb.in_v.ensure(mm * 4)) != hipSuccess ||
+        (e = c->bb.in_s.ensure(mm)) != hipSuccess)
+      return hip_fail(c, e, "upload alloc");
+    e = hipEventRecord(c->ev[0], c->st);
+    if (g->m && e == hipSuccess)
+      e = hipMemcpyAsync(c->bb.in_u.p, g->u, g->m * 4, hipMemcpyHostToDevice, c->st);
+    if (g->m && e == hipSuccess)
+      e = hipMemcpyAsync(c->bb.in_v.p, g->v, g->m * 4, hipMemcpyHostToDevice, c->st);
+    if (g->m && e == hipSuccess)
+      e = hipMemcpyAsync(c->bb.in_s.p, g->sign, g->m, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms_up, c->ev[0], c->ev[1]);
+    if (e != hipSuccess) return hip_fail(c, e, "upload");
+    du = c->bb.in_u.as<uint32_t>();
+    dv = c->bb.in_v.as<uint32_t>();
+    ds = c->bb.in_s.as<int8_t>();
+  }
+  std::string why;
+  e = hipEventRecord(c->ev[0], c->st);
+  if (e == hipSuccess) e = build_csr(du, dv, ds, g->m, g->n, nullptr, 0, c->bb, c->br, why, c->st);
+  if (e == hipErrorInvalidValue && !why.empty()) return fail(c, STC_E_ARG, why);
+  if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+  float ms_build = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms_build, c->ev[0], c->ev[1]);
+  if (e != hipSuccess) return hip_fail(c, e, "csr build");
+  c->g.n = c->br.n;
+  c->g.m = c->br.m;
+  c->g.off = c->bb.off.as<uint32_t>();
+  c->g.col = c->bb.col.as<uint32_t>();
+  c->g.src = c->bb.src.as<uint32_t>();
+  c->prepared = true;
+  std::memset(&c->stats, 0, sizeof(c->stats));
+  c->stats.n = c->br.n;
+  c->stats.m_in = c->br.m_in;
+  c->stats.m_oriented = c->br.m;
+  c->stats.max_out_degree = c->br.max_out;
+  c->stats.b_alg = c->br.b_alg;
+  c->stats.ms_upload = ms_up;
+  c->stats.ms_build = ms_build;
+  return STC_OK;
+}
+
+int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
+  if (!c || !out) return STC_E_ARG;
+  if (!c->prepared) return fail(c, STC_E_ARG, "no prepared graph");
+  (void)hipSetDevice(c->device);
+  int dispatch = opts ? opts->dispatch : STC_LEARNED;
+  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
+  bool classes = opts && opts->classes;
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+    return fail(c, STC_E_ARG, "bad dispatch");
+  c->hp.mode = dispatch;
+  c->hp.graph_feat[0] = (float)std::log2((double)c->g.n + 1.0);
+  c->hp.graph_feat[1] = (float)std::log2((double)c->g.m + 1.0);
+  hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "params upload");
+  std::vector<float> times, bins;
+  for (int r = 0; r < repeat; r++) {
+    e = hipEventRecord(c->ev[0], c->st);
+    if (e == hipSuccess)
+      e = enqueue_count(c->g, c->hp, c->dp, c->cb, classes, c->num_cus, c->st, c->ev[2]);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+    float ms = 0.f, msb = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&msb, c->ev[0], c->ev[2]);
+    if (e != hipSuccess) return hip_fail(c, e, "count");
+    times.push_back(ms);
+    bins.push_back(msb);
+  }
+  CountResult res;
+  e = read_count(c->cb, classes, res, c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "count readback");
+  if (classes) {
+    c->classes = {res.t[0], res.t[1], res.t[2], res.t[3]};
+    c->have_classes = true;
+    out->balanced = res.t[3] + res.t[1];
+    out->unbalanced = res.t[2] + res.t[0];
+  } else {
+    c->have_classes = false;
+    out->balanced = res.t[0];
+    out->unbalanced = res.t[1];
+  }
+  c->stats.ms_count = median(times);
+  c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->stats.ms_bin = bins.back();
+  c->stats.edges_skipped = res.edges_by_strat[S_SKIP];
+  c->stats.edges_merge = res.edges_by_strat[S_MERGE_LANE] + res.edges_by_strat[S_MERGE_WAVE];
+  c->stats.edges_hash = res.edges_by_strat[S_HASH];
+  c->stats.edges_bitmap = res.edges_by_strat[S_BITMAP];
+  c->stats.dispatch_used =
+      (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
+  return STC_OK;
+}
+
+int stc_count(stc_ctx *c, const stc_edges *g, const stc_opts *opts, stc_counts *out) {
+  if (!c || !out) return STC_E_ARG;
+  int rc = stc_prepare(c, g, 0);
+  if (rc) return rc;
+  return stc_count_prepared(c, opts, out);
+}
+
+int stc_count_device(stc_ctx *c, const stc_edges *g, const stc_opts *opts, stc_counts *out) {
+  if (!c || !out) return STC_E_ARG;
+  int rc = stc_prepare(c, g, 1);
+  if (rc) return rc;
+  return stc_count_prepared(c, opts, out);
+}
+
+int stc_last_classes(const stc_ctx *c, stc_classes *out) {
+  if (!c || !out) return STC_E_ARG;
+  if (!c->have_classes) return STC_E_ARG;
+  *out = c->classes;
+  return STC_OK;
+}
+
+int stc_get_stats(const stc_ctx *c, stc_stats *out) {
+  if (!c || !out) return STC_E_ARG;
+  *out = c->stats;
+  return STC_OK;
+}
+
+float stc_run(stc_ctx *c, const stc_edges *g, uint64_t *total_balanced,
+              uint64_t *total_unbalanced) {
+  stc_opts o{STC_LEARNED, 1, 0};
+  stc_counts r{0, 0};
+  int rc = stc_count(c, g, &o, &r);
+  if (rc) return (float)rc;
+  if (total_balanced) *total_balanced = r.balanced;
+  if (total_unbalanced) *total_unbalanced = r.unbalanced;
+  return c->stats.ms_count;
+}
+
+}  // extern "C"
+
+extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
+                               const stc_opts *opts, stc_counts *out) {
+  if (!c || (!graphs && k) || (!out && k)) return STC_E_ARG;
+  for (uint32_t i = 0; i < k; i++) {
+    int rc = stc_count(c, &graphs[i], opts, &out[i]);
+    if (rc) return rc;
+  }
+  return STC_OK;
+}

@@ -1,0 +1,189 @@
+// Internal interface between the host orchestration (api.cpp) and the HIP
+// kernels (csr_build.hip, count.hip, mlp.hip, batch.hip).  Not part of the
+// public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace stc {
+
+// ---- strategies -----------------------------------------------------------
+// Per edge bucket, the dispatcher picks one of these.  Merge runs edge-centric
+// (one lane, or one wave64, per oriented edge); hash and bitmap run
+// vertex-grouped (one workgroup stages the source's out-list in LDS and
+// streams the targets' out-lists against it).
+enum Strategy : uint8_t {
+  S_SKIP = 0,
+  S_MERGE_LANE = 1,  // one lane per edge, two-pointer merge
+  S_MERGE_WAVE = 2,  // one wave64 per edge, LDS-tiled cooperative merge
+  S_HASH = 3,        // LDS hash table of the source list
+  S_BITMAP = 4,      // LDS presence/sign bitmap over the source list's span
+  S_COUNT = 5
+};
+
+// Bucket of an oriented edge (u->v): (ceil_log2 of the usable part of
+// N+(u), ceil_log2 d+(v)), each clamped to 15.  The usable part of N+(u) is
+// the suffix after v: only out-neighbours ranked above v can close a
+// triangle with u->v.
+constexpr int kBucketSide = 16;
+constexpr int kBuckets = kBucketSide * kBucketSide;
+
+// Dispatcher inputs/outputs resident in device memory.
+constexpr int kMlpIn = 6;
+constexpr int kMlpHidden = 16;
+constexpr int kMlpOut = 3;  // merge, hash, bitmap
+struct MlpModel {
+  float mean[kMlpIn];
+  float inv_std[kMlpIn];
+  float w1[kMlpHidden][kMlpIn];
+  float b1[kMlpHidden];
+  float w2[kMlpOut][kMlpHidden];
+  float b2[kMlpOut];
+};
+
+enum DispatchMode : int {
+  D_LEARNED = 0,
+  D_FIXED = 1,
+  D_FORCE_MERGE = 2,
+  D_FORCE_HASH = 3,
+  D_FORCE_BITMAP = 4,
+  D_TABLE = 5  // use the strategy table given by the host (training/timing)
+};
+
+struct TuneParams {
+  // merge lane class: buckets whose longer side is <= 2^wave_min_log2 use one
+  // lane per edge, else one wave per edge
+  int merge_wave_min_log2[kBucketSide];  // indexed by log2 of shorter side
+  int hash_max_log2;      // hash only if the source list has <= 2^x entries
+  int bitmap_span_log2;   // bitmap span capacity (entries)
+  int hash_block;         // threads per workgroup for hash/bitmap
+  int fixed_skew_log2;    // fixed rule: skew >= 2^x -> hash
+  int fixed_hub_log2;     // fixed rule: d+(v) >= 2^x (or source) -> bitmap
+};
+
+struct DispatchParams {
+  int mode;
+  int have_model;
+  MlpModel model;
+  TuneParams tune;
+  uint8_t host_table[kBuckets];  // D_TABLE
+  float graph_feat[2];           // log2 n, log2 m of the graph
+};
+
+// Device view of the oriented CSR (rank space).
+struct DevCsr {
+  uint32_t n = 0;
+  uint64_t m = 0;
+  const uint32_t *off = nullptr;  // n + 1
+  const uint32_t *col = nullptr;  // m: (dst_rank << 1) | neg
+  const uint32_t *src = nullptr;  // m: src rank
+};
+
+// ---- device memory --------------------------------------------------------
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow to at least `need` bytes (contents not preserved).
+  hipError_t ensure(size_t need) {
+    if (need == 0) need = 256;
+    if (need <= bytes) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return e;
+    }
+    bytes = need;
+    return hipSuccess;
+  }
+  template <class T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+// Scratch for scans and sorts.
+struct ScanScratch {
+  DevBuf level[4];
+};
+
+// ---- launchers (return hipSuccess or the first error) -------------------
+hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
+                              ScanScratch &scr, hipStream_t s);
+// Stable LSD radix sort of 64-bit keys on the given 8-bit digit shifts
+// (ascending significance).  Result lands in *keys or *tmp; on return *keys
+// points at the sorted data.
+hipError_t radix_sort_u64(uint64_t **keys, uint64_t **tmp, uint64_t n,
+                          const std::vector<int> &shifts, DevBuf &hist,
+                          ScanScratch &scr, hipStream_t s);
+
+struct BuildBuffers {
+  DevBuf in_u, in_v, in_s;
+  DevBuf keyA, keyB, flag, pos;
+  DevBuf deg, vkeyA, vkeyB, rank, perm, graph_of;
+  DevBuf col, src, off, fdeg;
+  DevBuf hist, misc;
+  ScanScratch scr;
+};
+
+// Builds the rank-space oriented CSR from a device edge list.  vbase (k+1
+// entries, device) gives per-graph vertex ranges for a disjoint union of k
+// graphs (nullptr for one graph).
+struct BuildResult {
+  uint32_t n = 0;
+  uint64_t m_in = 0, m = 0;
+  uint32_t max_deg = 0, max_out = 0;
+  double b_alg = 0;
+};
+hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
+                     uint64_t m_in, uint32_t n, const uint32_t *vbase_dev,
+                     uint32_t k, BuildBuffers &bb, BuildResult &res,
+                     std::string &err, hipStream_t st);
+
+struct CountBuffers {
+  DevBuf gcount, lists, vflag, vlist;
+  ScanScratch scan;
+};
+
+struct CountResult {
+  uint64_t t[4] = {0, 0, 0, 0};  // t0..t3 when classes, else t[0]=bal t[1]=unbal
+  uint64_t edges_by_strat[S_COUNT] = {0, 0, 0, 0, 0};
+  uint8_t table[kBuckets];
+  uint32_t bucket_cnt[kBuckets];
+};
+
+// Enqueue binning + counting on `st` (no host sync inside).  hp is the host
+// copy of *dp (launch shapes come from its tune part).
+hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
+                         const DispatchParams *dp, CountBuffers &cb, bool classes,
+                         int num_cus, hipStream_t st, hipEvent_t ev_bin_done);
+// Read back totals and per-strategy sizes after the stream completes.
+hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r,
+                      hipStream_t st);
+size_t hash_lds_bytes(const TuneParams &t);
+size_t bitmap_lds_bytes(const TuneParams &t);
+
+// Persistent batched counting: one launch over the union CSR.
+hipError_t enqueue_batch_persistent(const DevCsr &g,
+                                    const DispatchParams *params_dev,
+                                    CountBuffers &cb, const uint32_t *vbase,
+                                    uint32_t k, uint64_t *per_graph,
+                                    int num_cus, hipStream_t st);
+
+// MLP training on device (full-batch gradient descent, one workgroup,
+// fixed reduction order -> deterministic).
+hipError_t train_mlp_device(const float *feat, const float *cost, int nsamp,
+                            int iters, float lr, float wd, MlpModel *model_io,
+                            hipStream_t st);
+
+}  // namespace stc

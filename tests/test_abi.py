@@ -1,0 +1,131 @@
+"""CPU-side checks of the native library: exports, host parser, generators.
+
+No compute call touches a GPU here; stc_create must report STC_E_NODEV on a
+machine without one.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import signed_triangle_counting_amd as stc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    names = set()
+    for h in ("stc.h", "stc_gen.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for mm in re.finditer(r"^[A-Za-z_][\w \t\*]*?\b(stc_\w+)\s*\(", txt, flags=re.M):
+            names.add(mm.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = stc.lib()
+    declared = header_functions()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(declared) == set(stc.EXPORTED)
+    out = subprocess.run(["nm", "-D", "--defined-only", stc.LIB_PATH], stdout=subprocess.PIPE,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (stc_\w+)", out))
+    assert set(declared) <= exported
+
+
+def test_no_gpu_reports_nodev():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(stc.StcError) as ei:
+        stc.Counter(0)
+    assert ei.value.code == stc.STC_E_NODEV
+
+
+PARSE_TEXTS = [
+    "0 1 1\n1 2 -1\n",
+    "# c\n% c\n3 4\n",
+    "1\t2\t-1\r\n2 3 +1\r\n",
+    "0 1 0\n1 2 -0\n2 3 7\n",
+    "5 5 1\n0 1 -1\n",
+    " 7 8 -1\n",
+    "0 1 -1 extra words\n",
+    "4294967297 3 1\n",
+    "\n\n\r\n0 1\n",
+    "0 1 1",
+    "3 3\n",
+    "",
+]
+
+
+@pytest.mark.parametrize("text", PARSE_TEXTS)
+def test_host_parser_matches_oracle(text):
+    g = stc.parse_graph(text)
+    n, u, v, s = oracle.parse(text)
+    assert g.n == n
+    assert g.u.tolist() == u.tolist() and g.v.tolist() == v.tolist()
+    assert g.sign.tolist() == s.tolist()
+
+
+def test_parser_threads_and_file_roundtrip(tmp_path):
+    g = stc.gen_workload(1)
+    p = str(tmp_path / "w1.txt")
+    g.write(p)
+    for threads in (1, 3, 8):
+        h = stc.load_graph(p, threads)
+        assert h.n == g.n and h.m == g.m
+        assert np.array_equal(h.u, g.u) and np.array_equal(h.v, g.v)
+        assert np.array_equal(h.sign, g.sign)
+    with open(p, "rb") as f:
+        n, u, v, s = oracle.parse(f.read())
+    assert n == g.n and np.array_equal(u, g.u) and np.array_equal(s, g.sign)
+
+
+def test_load_missing_file():
+    with pytest.raises(stc.StcError) as ei:
+        stc.load_graph("/nonexistent/graph.txt")
+    assert ei.value.code == stc.STC_E_IO
+
+
+def test_generators_deterministic_and_thread_independent():
+    a = stc.gen_rmat(14, 8, 0.57, 0.19, 0.19, 0.25, 99, threads=1)
+    b = stc.gen_rmat(14, 8, 0.57, 0.19, 0.19, 0.25, 99, threads=7)
+    assert np.array_equal(a.u, b.u) and np.array_equal(a.v, b.v)
+    assert np.array_equal(a.sign, b.sign)
+    assert not np.any(a.u == a.v)
+    e1 = stc.gen_er(500, 3000, 0.3, 5)
+    e2 = stc.gen_er(500, 3000, 0.3, 5)
+    assert np.array_equal(e1.u, e2.u) and np.array_equal(e1.sign, e2.sign)
+    key = np.minimum(e1.u, e1.v).astype(np.int64) * 500 + np.maximum(e1.u, e1.v)
+    assert len(np.unique(key)) == 3000        # G(n, m): distinct pairs
+    # duplicate lines of one edge carry one sign
+    key = np.minimum(a.u, a.v).astype(np.int64) << 32 | np.maximum(a.u, a.v)
+    order = np.argsort(key, kind="stable")
+    ks, ss = key[order], a.sign[order]
+    same = ks[1:] == ks[:-1]
+    assert np.all(ss[1:][same] == ss[:-1][same])
+
+
+def test_generator_parameters():
+    g = stc.gen_grid(30, 20, 0.5, 0.2, 3)
+    assert g.n == 600
+    diag = g.m - (30 * 19 + 29 * 20)
+    c = oracle.Csr(g.n, g.u, g.v, g.sign)
+    assert sum(c.classes().values()) == 2 * diag
+    h = stc.gen_hub(200000, 210000, 8, 1.2, 0.6, 2000, 0.1, 4)
+    assert h.m == 210000
+    deg = np.bincount(np.concatenate([h.u, h.v]), minlength=h.n)
+    top = np.sort(deg)[-8:][::-1]
+    assert top[0] > 40000 and top.sum() > 0.55 * 210000
+    b = stc.gen_batch(16, seed=3)
+    assert b.k == 16 and np.all(b.n >= 64) and np.all(b.n <= 65536)
+    for gi in range(b.k):
+        lo, hi = b.edge_off[gi], b.edge_off[gi + 1]
+        assert np.all(b.u[lo:hi] < b.n[gi]) and np.all(b.v[lo:hi] < b.n[gi])
+        assert not np.any(b.u[lo:hi] == b.v[lo:hi])

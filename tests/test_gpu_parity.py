@@ -1,0 +1,194 @@
+"""GPU parity: the HIP path, called through the C ABI, against the oracle and
+against the reference programs' golden counts.  Integer work: bit-exact.
+"""
+import itertools
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import signed_triangle_counting_amd as stc
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["learned", "fixed", "merge", "hash", "bitmap"]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def oracle_counts(n, u, v, s):
+    return oracle.Csr(n, u, v, s).merge()
+
+
+def random_graph(rng, n, m, p_neg=0.4, dup=0.0, loops=0.0):
+    u = rng.integers(0, n, m).astype(np.uint32)
+    v = rng.integers(0, n, m).astype(np.uint32)
+    s = np.where(rng.random(m) < p_neg, -1, 1).astype(np.int8)
+    if dup > 0:
+        k = int(m * dup)
+        idx = rng.integers(0, m, k)
+        u2, v2 = v[idx], u[idx]
+        u, v = np.concatenate([u, u2]), np.concatenate([v, v2])
+        s = np.concatenate([s, np.where(rng.random(k) < 0.5, -1, 1).astype(np.int8)])
+    if loops > 0:
+        k = int(m * loops)
+        w = rng.integers(0, n, k).astype(np.uint32)
+        u, v = np.concatenate([u, w]), np.concatenate([v, w])
+        s = np.concatenate([s, np.ones(k, np.int8)])
+    return u, v, s
+
+
+def test_small_golden_inputs(ctx, goldens):
+    for name, r in goldens["small"].items():
+        text = r.get("text", "")
+        g = stc.parse_graph(text)
+        for mode in MODES:
+            say("small", name, mode)
+            assert ctx.count(g, dispatch=mode) == (r["balanced"], r["unbalanced"]), (name, mode)
+        if "classes" in r:
+            ctx.count(g, classes=True)
+            assert ctx.classes() == r["classes"], name
+
+
+@pytest.mark.parametrize("wid", [1, 2, 3, 4])
+def test_workloads_vs_reference_goldens(ctx, goldens, wid):
+    r = goldens["workloads"][str(wid)]
+    say("workload", wid, "generate")
+    g = stc.gen_workload(wid)
+    say("workload", wid, "prepare n=%d lines=%d" % (g.n, g.m))
+    ctx.prepare(g)
+    st = ctx.stats()
+    assert st["m_oriented"] == r["m_oriented"]
+    want = (r["balanced"], r["unbalanced"])
+    for mode in MODES:
+        say("workload", wid, mode)
+        assert ctx.count_prepared(dispatch=mode) == want, mode
+    say("workload", wid, "classes")
+    ctx.count_prepared(dispatch="learned", classes=True)
+    assert ctx.classes() == r["classes"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_graphs_vs_oracle(ctx, seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(2, 3000))
+    m = int(rng.integers(1, 40 * n))
+    u, v, s = random_graph(rng, n, m, dup=0.2, loops=0.05)
+    want = oracle_counts(n, u, v, s)
+    g = stc.Graph(u=u, v=v, sign=s, n=n)
+    for mode in MODES:
+        say("random", seed, n, len(u), mode)
+        assert ctx.count(g, dispatch=mode) == want, mode
+
+
+@pytest.mark.parametrize("scale,ef", [(10, 16), (12, 32), (14, 24), (16, 16)])
+def test_rmat_vs_oracle(ctx, scale, ef):
+    g = stc.gen_rmat(scale, ef, 0.57, 0.19, 0.19, 0.3, 77 + scale)
+    want = oracle_counts(g.n, g.u, g.v, g.sign)
+    ctx.prepare(g)
+    for mode in MODES:
+        say("rmat", scale, ef, mode)
+        assert ctx.count_prepared(dispatch=mode) == want, mode
+
+
+def test_dense_graph_long_lists(ctx):
+    """Out-lists beyond one wave, beyond the LDS hash cap and wide spans."""
+    rng = np.random.default_rng(5)
+    n = 6000
+    pairs = np.array(list(itertools.combinations(range(300), 2)), np.uint32)
+    u = np.concatenate([pairs[:, 0], rng.integers(0, n, 200000).astype(np.uint32)])
+    v = np.concatenate([pairs[:, 1], rng.integers(0, n, 200000).astype(np.uint32)])
+    s = np.where(rng.random(len(u)) < 0.35, -1, 1).astype(np.int8)
+    want = oracle_counts(n, u, v, s)
+    g = stc.Graph(u=u, v=v, sign=s, n=n)
+    ctx.prepare(g)
+    for mode in MODES:
+        say("dense", mode)
+        assert ctx.count_prepared(dispatch=mode) == want, mode
+
+
+@pytest.mark.parametrize("n", [3, 5, 64, 200])
+def test_complete_graphs(ctx, n):
+    pairs = np.array(list(itertools.combinations(range(n), 2)), np.uint32)
+    c3 = math.comb(n, 3)
+    for sgn, want in ((1, (c3, 0)), (-1, (0, c3))):
+        g = stc.Graph(u=pairs[:, 0], v=pairs[:, 1], sign=np.full(len(pairs), sgn, np.int8), n=n)
+        for mode in MODES:
+            say("K", n, sgn, mode)
+            assert ctx.count(g, dispatch=mode) == want
+
+
+def test_edge_cases(ctx):
+    say("empty")
+    assert ctx.count(stc.Graph(u=[], v=[], sign=[], n=0)) == (0, 0)
+    assert ctx.count(stc.Graph(u=[], v=[], sign=[], n=10)) == (0, 0)
+    say("loops only")
+    assert ctx.count(stc.Graph(u=[1, 2, 3], v=[1, 2, 3], sign=[1, -1, 1], n=4)) == (0, 0)
+    say("single edge")
+    assert ctx.count(stc.Graph(u=[0], v=[1], sign=[-1], n=2)) == (0, 0)
+    say("conflicting duplicates: first line wins")
+    g = stc.Graph(u=[0, 1, 2, 1, 0], v=[1, 2, 0, 0, 2], sign=[1, 1, -1, -1, 1], n=3)
+    assert ctx.count(g) == (0, 1)
+    say("id >= n rejected")
+    with pytest.raises(stc.StcError) as ei:
+        ctx.count(stc.Graph(u=[0, 5], v=[1, 2], sign=[1, 1], n=3))
+    assert ei.value.code == stc.STC_E_ARG
+    say("context still usable after an error")
+    assert ctx.count(stc.Graph(u=[0, 1, 0], v=[1, 2, 2], sign=[1, 1, 1], n=3)) == (1, 0)
+
+
+def test_classes_vs_oracle(ctx):
+    rng = np.random.default_rng(9)
+    u, v, s = random_graph(rng, 2000, 60000, dup=0.1)
+    want = oracle.Csr(2000, u, v, s).classes()
+    g = stc.Graph(u=u, v=v, sign=s, n=2000)
+    for mode in MODES:
+        say("classes", mode)
+        b, un = ctx.count(g, dispatch=mode, classes=True)
+        assert ctx.classes() == want
+        assert (b, un) == (want["t3"] + want["t1"], want["t2"] + want["t0"])
+
+
+def test_repeat_and_device_input(ctx):
+    import torch
+    g = stc.gen_workload(1)
+    want = oracle_counts(g.n, g.u, g.v, g.sign)
+    say("repeat")
+    assert ctx.count(g, repeat=5) == want
+    st = ctx.stats()
+    assert st["ms_count"] > 0 and st["m_oriented"] == 100000
+    say("device input")
+    du = torch.from_numpy(g.u.astype(np.int32)).cuda()
+    dv = torch.from_numpy(g.v.astype(np.int32)).cuda()
+    ds = torch.from_numpy(g.sign.copy()).cuda()
+    torch.cuda.synchronize()
+    got = ctx.count_device(g.n, g.m, du.data_ptr(), dv.data_ptr(), ds.data_ptr())
+    assert got == want
+
+
+def test_reference_operator_shape(ctx):
+    g = stc.gen_workload(1)
+    res = stc.signed_tc.result_t()
+    ms = stc.signed_tc.run(g, stc.signed_tc.param_t(), res, context=ctx)
+    assert ms > 0
+    assert (res.total_balanced, res.total_unbalanced) == oracle_counts(g.n, g.u, g.v, g.sign)
+
+
+def test_cli_output(tmp_path, goldens):
+    r = goldens["small"]["k4_mixed"]
+    p = tmp_path / "k4.txt"
+    p.write_text(r["text"])
+    say("cli")
+    out = subprocess.run([stc.CLI_PATH, str(p)], stdout=subprocess.PIPE, text=True,
+                         timeout=120).stdout
+    assert "Balanced   = %d" % r["balanced"] in out
+    assert "Unbalanced = %d" % r["unbalanced"] in out
+    assert "Total      = %d" % (r["balanced"] + r["unbalanced"]) in out
+    assert "[Triangle Counting Execution] took" in out
+    bad = subprocess.run([stc.CLI_PATH], stdout=subprocess.PIPE, text=True, timeout=60)
+    assert bad.returncode == 1 and "Usage" in bad.stdout
