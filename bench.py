@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the signed triangle counting hot path on one MI355X.
+
+Workload (default): RMAT scale 22, edge factor 16, (a,b,c,d) =
+(0.57,0.19,0.19,0.05), signs -1 with probability 0.25 -- 67,108,864 generated
+lines, 64,155,555 oriented edges after dedupe.  The graph is generated on the
+host, uploaded and turned into the oriented CSR on the device once (reported
+as build_ms); a step is one pass of the counting phase (binning + every
+intersection kernel) over that device-resident CSR.
+
+The JSON line carries:
+  value       oriented edges / s of the counting phase (wall clock over the
+              timed steps, bracketed by device synchronisation)
+  roofline    the dominant kernel's algorithmic bytes (4(d+(u)+d+(v)) per
+              oriented edge it counted) / its mean hipEvent duration, against
+              the 8 TB/s HBM peak; traffic = FETCH_SIZE + WRITE_SIZE bytes per
+              launch from the committed rocprofv3 PMC summary, if present
+  cpu_baseline the CPU oracle's merge count (a restatement of the reference's
+              OpenMP merge path) timed on this host over a bounded sample
+  learned_vs_fixed  counting time of the fixed-threshold dispatch over the
+              learned dispatch on the hub-dominated graph (18M vertices)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+KERNELS = ["merge_lane", "merge_wave", "hash", "bitmap"]
+WORKLOADS = {1: "er_10k_100k", 2: "grid_960x960_diag", 3: "rmat22_ef16", 4: "hub_18m"}
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", type=int, default=3)
+    ap.add_argument("--dispatch", default="learned")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hub", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def goldens():
+    p = os.path.join(ROOT, "tests", "golden", "goldens.json")
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except OSError:
+        return {}
+
+
+def traffic_for(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            t = json.load(f)
+        rec = t.get(workload, {}).get(kernel)
+        return rec["bytes_per_launch"] if rec else None
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def cpu_baseline(g, budget_s):
+    """The oracle's merge count (tc_merge_sign_enu.cpp:131-145 restated) over
+    every stride-th source vertex, stride chosen so the sample takes about
+    budget_s seconds; returns oriented edges per second."""
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.time()
+    csr = oracle.Csr(g.n, g.u, g.v, g.sign, threads)
+    build_s = time.time() - t0
+    probe = csr.merge_sample(1024, 0, threads)
+    per_vertex_s = probe["seconds"] / max(1, (g.n + 1023) // 1024)
+    est_full = per_vertex_s * g.n
+    stride = max(1, int(est_full / budget_s) + 1)
+    s = csr.merge_sample(stride, 1 % stride, threads)
+    return {
+        "value": s["edges"] / s["seconds"] if s["seconds"] > 0 else None,
+        "unit": "oriented edges/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "oracle merge count over source vertices v %% %d == %d (%d of %d oriented "
+                  "edges, %.1f s); oracle CSR build %.1f s not included"
+                  % (stride, 1 % stride, s["edges"], csr.m, s["seconds"], build_s),
+    }
+
+
+def hub_learned_vs_fixed(ctx, stc, steps):
+    g = stc.gen_workload(4)
+    ctx.prepare(g)
+    out = {"workload": WORKLOADS[4], "m_oriented": ctx.stats()["m_oriented"]}
+    res = {}
+    for mode in ("fixed", "learned"):
+        ctx.count_prepared(dispatch=mode, repeat=2)
+        res[mode] = ctx.count_prepared(dispatch=mode, repeat=steps)
+        out[mode + "_ms"] = round(ctx.stats()["ms_count"], 4)
+    out["exact_equal"] = res["fixed"] == res["learned"]
+    out["speedup"] = round(out["fixed_ms"] / out["learned_ms"], 3) if out["learned_ms"] else None
+    return out
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build counts one graph (or one batch) per "
+              "GPU and has no multi-GPU path; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import signed_triangle_counting_amd as stc
+
+    wname = WORKLOADS[args.workload]
+    g = stc.gen_workload(args.workload)
+    ctx = stc.Counter(0)
+    ctx.prepare(g)
+    st0 = ctx.stats()
+    if args.warmup > 0:
+        ctx.count_prepared(dispatch=args.dispatch, repeat=args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    counts = ctx.count_prepared(dispatch=args.dispatch, repeat=args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    ms_per_step = 1e3 * wall / args.steps
+    m_plus = st["m_oriented"]
+    gold = goldens().get("workloads", {}).get(str(args.workload))
+    exact = None if gold is None else (counts == (gold["balanced"], gold["unbalanced"]))
+
+    k = max(range(4), key=lambda i: st["ms_kernel"][i])
+    kname = KERNELS[k]
+    kms = st["ms_kernel"][k]
+    achieved = st["bytes_kernel"][k] / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = {
+        "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+        "traffic": traffic_for(kname, wname),
+        "kernel_ms": round(kms, 4), "kernel_bytes": st["bytes_kernel"][k],
+        "count_phase_achieved": round(st["b_alg"] / (st["ms_count"] * 1e-3) / 1e9, 1),
+        "count_phase_frac": round(st["b_alg"] / (st["ms_count"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+    }
+    line = {
+        "metric": "oriented_edges_per_s", "value": m_plus / (ms_per_step * 1e-3),
+        "unit": "oriented edges/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": wname, "n": st["n"], "lines": st["m_in"],
+                   "m_oriented": m_plus, "dispatch": args.dispatch,
+                   "parallelism": "single GPU"},
+        "counts": {"balanced": counts[0], "unbalanced": counts[1], "exact_vs_reference": exact},
+        "ms_count_event_median": round(st["ms_count"], 4), "ms_bin": round(st["ms_bin"], 4),
+        "ms_kernels": dict(zip(KERNELS, [round(x, 4) for x in st["ms_kernel"]])),
+        "build_ms": round(st0["ms_build"], 3), "upload_ms": round(st0["ms_upload"], 3),
+        "b_alg_bytes": st["b_alg"],
+        "roofline": roofline,
+        "edges_by_strategy": {"merge": st["edges_merge"], "hash": st["edges_hash"],
+                              "bitmap": st["edges_bitmap"], "skipped": st["edges_skipped"]},
+        "model_loaded": ctx.model_status == 0,
+    }
+    if not args.no_hub:
+        line["learned_vs_fixed"] = hub_learned_vs_fixed(ctx, stc, max(3, args.steps // 2))
+    ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

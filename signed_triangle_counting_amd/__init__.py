@@ -90,10 +90,15 @@ class Stats(C.Structure):
                 ("ms_count", C.c_float), ("ms_count_min", C.c_float),
                 ("ms_bin", C.c_float), ("edges_merge", C.c_uint64),
                 ("edges_hash", C.c_uint64), ("edges_bitmap", C.c_uint64),
-                ("edges_skipped", C.c_uint64), ("dispatch_used", C.c_int)]
+                ("edges_skipped", C.c_uint64), ("dispatch_used", C.c_int),
+                ("ms_kernel", C.c_float * 4), ("bytes_kernel", C.c_double * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = list(v) if k in ("ms_kernel", "bytes_kernel") else v
+        return d
 
 
 _lib_handle = None

@@ -39,6 +39,7 @@ struct stc_ctx {
   stc_classes classes{};
   bool have_classes = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t marks[kMarks] = {};
   DevBuf batch_counts, batch_vbase;
 };
 
@@ -59,7 +60,7 @@ int hip_fail(stc_ctx *c, hipError_t e, const char *where) {
 
 void default_tune(TuneParams &t) {
   for (int i = 0; i < kBucketSide; i++) t.merge_wave_min_log2[i] = 5;
-  t.hash_max_log2 = 11;
+  t.hash_max_log2 = 10;
   t.bitmap_span_log2 = 17;
   t.hash_block = 256;
   t.fixed_skew_log2 = 2;
@@ -215,6 +216,7 @@ int stc_create(int device, const char *model_dir, stc_ctx **out) {
     e = hipStreamCreate(&c->st);
   }
   for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < kMarks && e == hipSuccess; i++) e = hipEventCreate(&c->marks[i]);
   if (e == hipSuccess) e = hipMalloc(&c->dp, sizeof(DispatchParams));
   if (e != hipSuccess) {
     int rc = hip_fail(c, e, "stc_create");
@@ -242,6 +244,8 @@ void stc_destroy(stc_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : c->marks)
     if (ev) (void)hipEventDestroy(ev);
   if (c->dp) (void)hipFree(c->dp);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -322,16 +326,21 @@ int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
   c->hp.graph_feat[1] = (float)std::log2((double)c->g.m + 1.0);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   if (e != hipSuccess) return hip_fail(c, e, "params upload");
-  std::vector<float> times, bins;
+  std::vector<float> times, bins, kt[4];
   for (int r = 0; r < repeat; r++) {
     e = hipEventRecord(c->ev[0], c->st);
     if (e == hipSuccess)
-      e = enqueue_count(c->g, c->hp, c->dp, c->cb, classes, c->num_cus, c->st, c->ev[2]);
+      e = enqueue_count(c->g, c->hp, c->dp, c->cb, classes, c->num_cus, c->st, c->marks);
     if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
     if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
     float ms = 0.f, msb = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&msb, c->ev[0], c->ev[2]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&msb, c->marks[0], c->marks[1]);
+    for (int k = 0; k < 4 && e == hipSuccess; k++) {
+      float t = 0.f;
+      e = hipEventElapsedTime(&t, c->marks[k + 1], c->marks[k + 2]);
+      kt[k].push_back(t);
+    }
     if (e != hipSuccess) return hip_fail(c, e, "count");
     times.push_back(ms);
     bins.push_back(msb);
@@ -356,6 +365,10 @@ int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
   c->stats.edges_merge = res.edges_by_strat[S_MERGE_LANE] + res.edges_by_strat[S_MERGE_WAVE];
   c->stats.edges_hash = res.edges_by_strat[S_HASH];
   c->stats.edges_bitmap = res.edges_by_strat[S_BITMAP];
+  for (int k = 0; k < 4; k++) {
+    c->stats.ms_kernel[k] = median(kt[k]);
+    c->stats.bytes_kernel[k] = 4.0 * (double)res.words_by_strat[k + 1];
+  }
   c->stats.dispatch_used =
       (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
   return STC_OK;

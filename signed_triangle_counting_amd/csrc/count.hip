@@ -33,11 +33,15 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 struct Ctrl {
   uint32_t bucket_cnt[kBuckets];
   uint32_t done;
-  uint32_t len[4];   // merge-lane list, merge-wave list, hash vertices, bitmap vertices
+  uint32_t len[6];   // merge-lane list, merge-wave list, hash vertices (workgroup),
+                     // bitmap vertices, hash vertices (wave)
   uint32_t work[8];  // dynamic work counters
-  uint32_t pad[3];
+  uint32_t pad[1];
+  uint32_t queue[4][8][32];          // XCD work queues: one counter per XCD,
+                                     // 128 bytes apart
   unsigned long long t[4];           // class counters
   unsigned long long strat_edges[8]; // oriented edges per strategy
+  unsigned long long strat_words[8]; // sum of d+(u) + d+(v) per strategy
   uint8_t table[kBuckets];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl alignment");
@@ -170,6 +174,35 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst, uint64_t v) {
   if (__lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
+// ---- XCD-aware work queue ---------------------------------------------------
+// `len` items are split into 8 contiguous ranges, one per XCD (workgroups are
+// dispatched round-robin over the 8 XCDs of an MI355X, so blockIdx % 8 names
+// the XCD and a range's vertices share that XCD's L2).  Each range has its own
+// counter on its own 128-byte line and is drained from its end (highest ranks,
+// the heaviest vertices, first); a drained range sends the caller on to the
+// next XCD's range.  Returns false when every range is empty.
+struct QueueCursor {
+  uint32_t q, tried;
+};
+
+__device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uint32_t chunk,
+                                          QueueCursor &cur, uint32_t &first,
+                                          uint32_t &count) {
+  while (cur.tried < 8) {
+    const uint32_t lo = (uint32_t)(((uint64_t)len * cur.q) >> 3);
+    const uint32_t hi = (uint32_t)(((uint64_t)len * (cur.q + 1)) >> 3);
+    const uint32_t k = atomicAdd(&ctr[cur.q][0], chunk);
+    if (k < hi - lo) {
+      count = min(chunk, hi - lo - k);
+      first = hi - k - count;  // items [first, first + count), taken from the top
+      return true;
+    }
+    cur.q = (cur.q + 1) & 7u;
+    cur.tried++;
+  }
+  return false;
+}
+
 // ---- binning ----------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bin_hist(DevCsr g, Ctrl *ctrl,
                                                   const DispatchParams *P) {
@@ -207,58 +240,124 @@ __global__ __launch_bounds__(256) void k_bin_hist(DevCsr g, Ctrl *ctrl,
   }
 }
 
+// Edge lists are written per workgroup: workgroup b owns the segment
+// [b * seg, (b + 1) * seg) of each list (seg = the most edges one workgroup
+// visits) and appends through an LDS counter; k_list_compact then packs the
+// segments.  No global atomic sits on a hot address.
+__device__ __forceinline__ uint32_t lds_append(bool pred, uint32_t *s_len) {
+  uint64_t mask = __ballot(pred);
+  uint32_t base = 0;
+  if (__lane_id() == 0 && mask) base = atomicAdd(s_len, (uint32_t)__popcll(mask));
+  base = __shfl(base, 0, 64);
+  return base + (uint32_t)__popcll(mask & lanemask_lt());
+}
+
 __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
-                                                     uint32_t *__restrict__ lane_list,
-                                                     uint32_t *__restrict__ wave_list,
+                                                     uint32_t *__restrict__ lane_seg,
+                                                     uint32_t *__restrict__ wave_seg,
+                                                     uint32_t seg,
+                                                     uint32_t *__restrict__ blk_cnt,
                                                      uint32_t *__restrict__ vflag) {
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
+  __shared__ uint32_t s_len[2];
   const int tid = threadIdx.x;
   table[tid] = ctrl->table[tid];
-  if (tid == 0) tune = P->tune;
+  if (tid == 0) {
+    tune = P->tune;
+    s_len[0] = s_len[1] = 0;
+  }
   __syncthreads();
-  uint64_t cnt_s[S_COUNT] = {0, 0, 0, 0, 0};
+  uint32_t cnt_s[S_COUNT] = {0, 0, 0, 0, 0};
+  uint64_t words_s[S_COUNT] = {0, 0, 0, 0, 0};
+  uint32_t *lane_out = lane_seg + (uint64_t)blockIdx.x * seg;
+  uint32_t *wave_out = wave_seg + (uint64_t)blockIdx.x * seg;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t mround = (g.m + 63) & ~63ull;
+  const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
     uint8_t s = S_SKIP;
-    uint32_t a = 0;
-    if (e < g.m) {
+    uint32_t a = 0, words = 0;
+    const bool valid = e < g.m;
+    if (valid) {
       a = g.src[e];
       uint32_t b = g.col[e] >> 1;
       uint32_t la = g.off[a], ea = g.off[a + 1];
       uint32_t da = ea - (uint32_t)e - 1u;
       uint32_t db = g.off[b + 1] - g.off[b];
+      words = (ea - la) + db;
       if (da && db) {
         uint8_t s0 = table[bucket_of(da, db)];
         uint32_t span = s0 == S_BITMAP ? span_of(g.col, la, ea) : 0u;
         s = edge_strategy(table, tune, ea - la, span, da, db);
       }
     }
-    uint32_t p = wave_append(s == S_MERGE_LANE, &ctrl->len[0]);
-    if (s == S_MERGE_LANE) lane_list[p] = (uint32_t)e;
-    p = wave_append(s == S_MERGE_WAVE, &ctrl->len[1]);
-    if (s == S_MERGE_WAVE) wave_list[p] = (uint32_t)e;
+    uint32_t p = lds_append(s == S_MERGE_LANE, &s_len[0]);
+    if (s == S_MERGE_LANE) lane_out[p] = (uint32_t)e;
+    p = lds_append(s == S_MERGE_WAVE, &s_len[1]);
+    if (s == S_MERGE_WAVE) wave_out[p] = (uint32_t)e;
     if (s == S_HASH || s == S_BITMAP) {
       uint32_t bit = s == S_HASH ? 1u : 2u;
       if (!(vflag[a] & bit)) atomicOr(&vflag[a], bit);
     }
 #pragma unroll
-    for (int k = 1; k < S_COUNT; k++) cnt_s[k] += __popcll(__ballot(s == k));
-    cnt_s[0] += __popcll(__ballot(e < g.m && s == S_SKIP));
+    for (int k = 0; k < S_COUNT; k++) {
+      const bool hit = valid && s == k;
+      cnt_s[k] += hit;
+      words_s[k] += hit ? words : 0u;
+    }
   }
 #pragma unroll
-  for (int k = 0; k < S_COUNT; k++) wave_add(&ctrl->strat_edges[k], cnt_s[k]);
+  for (int k = 0; k < S_COUNT; k++) {
+    uint64_t c = cnt_s[k], w = words_s[k];
+    for (int d = 32; d > 0; d >>= 1) {
+      c += __shfl_down(c, d, 64);
+      w += __shfl_down(w, d, 64);
+    }
+    wave_add(&ctrl->strat_edges[k], c);
+    wave_add(&ctrl->strat_words[k], w);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    blk_cnt[blockIdx.x] = s_len[0];
+    blk_cnt[gridDim.x + blockIdx.x] = s_len[1];
+  }
 }
 
-// Vertices with hash / bitmap work, in ascending rank, then flags reset.
-__global__ void k_vflag_split(uint32_t *__restrict__ vflag, uint32_t n,
-                              uint32_t *__restrict__ fh, uint32_t *__restrict__ fb) {
+// Pack the per-workgroup segments: blk_pos holds the exclusive scan of
+// blk_cnt (2 * nblk entries: merge-lane counts, then merge-wave counts).
+__global__ __launch_bounds__(256) void k_list_compact(const uint32_t *__restrict__ lane_seg,
+                                                      const uint32_t *__restrict__ wave_seg,
+                                                      uint32_t seg,
+                                                      const uint32_t *__restrict__ blk_cnt,
+                                                      const uint32_t *__restrict__ blk_pos,
+                                                      uint32_t nblk,
+                                                      uint32_t *__restrict__ lane_list,
+                                                      uint32_t *__restrict__ wave_list,
+                                                      Ctrl *ctrl) {
+  const uint32_t b = blockIdx.x % nblk;
+  const bool lanes = blockIdx.x < nblk;
+  const uint32_t *src = (lanes ? lane_seg : wave_seg) + (uint64_t)b * seg;
+  uint32_t *dst = lanes ? lane_list : wave_list;
+  const uint32_t cnt = blk_cnt[blockIdx.x];
+  const uint32_t base = blk_pos[blockIdx.x] - (lanes ? 0u : blk_pos[nblk]);
+  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) dst[base + i] = src[i];
+  if (blockIdx.x == nblk - 1 && threadIdx.x == 0) ctrl->len[0] = blk_pos[nblk - 1] + cnt;
+  if (blockIdx.x == 2 * nblk - 1 && threadIdx.x == 0)
+    ctrl->len[1] = blk_pos[2 * nblk - 1] + cnt - blk_pos[nblk];
+}
+
+// Vertices with hash / bitmap work, in ascending rank: flags, then lists.
+// Hash sources with at most wave_max out-neighbours go to the wave list.
+__global__ void k_vflag_split(uint32_t *__restrict__ vflag, const uint32_t *__restrict__ off,
+                              uint32_t n, uint32_t wave_max, uint32_t *__restrict__ fh,
+                              uint32_t *__restrict__ fb, uint32_t *__restrict__ fw) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x) {
     uint32_t f = vflag[v];
-    fh[v] = f & 1u;
+    uint32_t small = (off[v + 1] - off[v]) <= wave_max;
+    fh[v] = (f & 1u) & (small ^ 1u);
+    fw[v] = (f & 1u) & small;
     fb[v] = (f >> 1) & 1u;
     if (f) vflag[v] = 0;
   }
@@ -267,16 +366,20 @@ __global__ void k_vflag_split(uint32_t *__restrict__ vflag, uint32_t n,
 __global__ void k_vlist_scatter(const uint32_t *__restrict__ fh,
                                 const uint32_t *__restrict__ ph,
                                 const uint32_t *__restrict__ fb,
-                                const uint32_t *__restrict__ pb, uint32_t n,
+                                const uint32_t *__restrict__ pb,
+                                const uint32_t *__restrict__ fw,
+                                const uint32_t *__restrict__ pw, uint32_t n,
                                 uint32_t *__restrict__ vh, uint32_t *__restrict__ vb,
-                                Ctrl *ctrl) {
+                                uint32_t *__restrict__ vw, Ctrl *ctrl) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x) {
     if (fh[v]) vh[ph[v]] = v;
     if (fb[v]) vb[pb[v]] = v;
+    if (fw[v]) vw[pw[v]] = v;
     if (v == n - 1) {
       ctrl->len[2] = ph[v] + fh[v];
       ctrl->len[3] = pb[v] + fb[v];
+      ctrl->len[4] = pw[v] + fw[v];
     }
   }
 }
@@ -354,32 +457,34 @@ __global__ __launch_bounds__(256) void k_merge_lane(DevCsr g, Ctrl *ctrl,
 }
 
 // ---- merge: one wave64 per edge, 64x64 tiles -----------------------------
+// A = the part of N+(u) after v, B = N+(v).  Each lane holds one element of the
+// current A tile and one of the current B tile; a 6-step ds_bpermute binary
+// search finds each A element in the B tile; the tile whose last element is
+// smaller advances (both on a tie).  The tile after each current tile is
+// loaded one step ahead so its latency overlaps the search.
 template <bool CLS>
 __global__ __launch_bounds__(256) void k_merge_wave(DevCsr g, Ctrl *ctrl,
                                                     const uint32_t *__restrict__ list) {
   const uint32_t len = ctrl->len[1];
   const uint32_t *__restrict__ col = g.col;
-  const int lane = __lane_id();
+  const uint32_t lane = __lane_id();
   const uint32_t nw = gridDim.x * (blockDim.x >> 6);
   Acc<CLS> acc;
   for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < len; i += nw) {
     const uint32_t e = list[i];
     const uint32_t w = col[e];
     const uint32_t a = g.src[e], b = w >> 1, sab = w & 1u;
-    uint32_t ia = e + 1, ea = g.off[a + 1];
-    uint32_t ib = g.off[b], eb = g.off[b + 1];
-    uint32_t xa = kEmpty, yb = kEmpty;
-    uint32_t nA = 0, nB = 0;
-    bool needA = true, needB = true;
-    while (ia < ea && ib < eb) {
-      if (needA) {
-        nA = min(64u, ea - ia);
-        xa = (uint32_t)lane < nA ? col[ia + lane] : kEmpty;
-      }
-      if (needB) {
-        nB = min(64u, eb - ib);
-        yb = (uint32_t)lane < nB ? col[ib + lane] : kEmpty;
-      }
+    uint32_t ia = e + 1;
+    const uint32_t ea = g.off[a + 1];
+    uint32_t ib = g.off[b];
+    const uint32_t eb = g.off[b + 1];
+    uint32_t nA = min(64u, ea - ia), nB = min(64u, eb - ib);
+    uint32_t xa = lane < nA ? col[ia + lane] : kEmpty;
+    uint32_t yb = lane < nB ? col[ib + lane] : kEmpty;
+    uint32_t ia2 = ia + nA, ib2 = ib + nB;
+    uint32_t xa2 = ia2 + lane < ea ? col[ia2 + lane] : kEmpty;
+    uint32_t yb2 = ib2 + lane < eb ? col[ib2 + lane] : kEmpty;
+    while (true) {
       // lower bound of xa among the 64 B entries (padding sorts last)
       const uint32_t key = xa >> 1;
       uint32_t pos = 0;
@@ -388,86 +493,246 @@ __global__ __launch_bounds__(256) void k_merge_wave(DevCsr g, Ctrl *ctrl,
         uint32_t cand = (uint32_t)__shfl((int)yb, (int)(pos + step - 1), 64);
         if ((cand >> 1) < key) pos += step;
       }
-      uint32_t hit = (uint32_t)__shfl((int)yb, (int)min(pos, 63u), 64);
-      bool found = (uint32_t)lane < nA && pos < nB && (hit >> 1) == key;
+      const uint32_t hit = (uint32_t)__shfl((int)yb, (int)min(pos, 63u), 64);
+      const bool found = lane < nA && pos < nB && (hit >> 1) == key;
       acc.add(found, sab + (xa & 1u) + (hit & 1u));
-      uint32_t lastA = (uint32_t)__shfl((int)xa, (int)(nA - 1), 64) >> 1;
-      uint32_t lastB = (uint32_t)__shfl((int)yb, (int)(nB - 1), 64) >> 1;
-      needA = lastA <= lastB;
-      needB = lastB <= lastA;
-      if (needA) ia += nA;
-      if (needB) ib += nB;
+      const uint32_t lastA = (uint32_t)__shfl((int)xa, (int)(nA - 1), 64) >> 1;
+      const uint32_t lastB = (uint32_t)__shfl((int)yb, (int)(nB - 1), 64) >> 1;
+      const bool needA = lastA <= lastB, needB = lastB <= lastA;
+      if (needA) {
+        ia = ia2;
+        if (ia >= ea) break;
+        nA = min(64u, ea - ia);
+        xa = xa2;
+        ia2 = ia + nA;
+        xa2 = ia2 + lane < ea ? col[ia2 + lane] : kEmpty;
+      }
+      if (needB) {
+        ib = ib2;
+        if (ib >= eb) break;
+        nB = min(64u, eb - ib);
+        yb = yb2;
+        ib2 = ib + nB;
+        yb2 = ib2 + lane < eb ? col[ib2 + lane] : kEmpty;
+      }
     }
   }
   acc.flush(ctrl);
 }
 
-// ---- vertex-grouped: LDS hash table of N+(u) ------------------------------
+// ---- vertex-grouped kernels -------------------------------------------------
 __device__ __forceinline__ uint32_t fib_hash(uint32_t r, int shift) {
   return (r * 0x9E3779B1u) >> shift;
 }
 
+// Wave-scope LDS ordering for the wave-per-vertex variant (no workgroup
+// barrier is needed when one wave owns an LDS region).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kUnroll = 4;  // independent 64-element loads in flight per wave
+
+// Stream N+(b) = col[pb, eb) through `probe` (one element per lane, kUnroll
+// chunks of 64 loaded before any is probed) and accumulate the classes.
+template <bool CLS, class Probe>
+__device__ __forceinline__ void stream_list(const uint32_t *__restrict__ col, uint32_t pb,
+                                            uint32_t eb, uint32_t sab, const Probe &probe,
+                                            Acc<CLS> &acc) {
+  const uint32_t lane = __lane_id();
+  for (uint32_t base = pb; base < eb; base += 64u * kUnroll) {
+    uint32_t y[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      uint32_t idx = base + u * 64u + lane;
+      y[u] = idx < eb ? col[idx] : kEmpty;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      if (base + u * 64u >= eb) break;  // wave-uniform
+      const bool valid = base + u * 64u + lane < eb;
+      uint32_t t = 0;
+      const bool found = probe(y[u], valid, t);
+      acc.add(found, sab + (y[u] & 1u) + (t & 1u));
+    }
+  }
+}
+
+// Walk the out-edges j = first, first+step, ... of source a (ending at ea),
+// prefetching the next edge's target offsets while the current one streams.
+template <class Body>
+__device__ __forceinline__ void walk_edges(const DevCsr &g, uint32_t first, uint32_t ea,
+                                           uint32_t step, const Body &body) {
+  uint32_t j = first;
+  uint32_t w = 0, pb = 0, eb = 0;
+  if (j < ea) {
+    w = g.col[j];
+    pb = g.off[w >> 1];
+    eb = g.off[(w >> 1) + 1];
+  }
+  while (j < ea) {
+    const uint32_t jn = j + step;
+    uint32_t wn = 0, pbn = 0, ebn = 0;
+    if (jn < ea) {
+      wn = g.col[jn];
+      pbn = g.off[wn >> 1];
+      ebn = g.off[(wn >> 1) + 1];
+    }
+    body(j, w, pb, eb);
+    j = jn;
+    w = wn;
+    pb = pbn;
+    eb = ebn;
+  }
+}
+
+// Linear-probe lookup of rank yr in an LDS table of `mask + 1` slots.
+__device__ __forceinline__ bool hash_find(const uint32_t *T, uint32_t mask, int shift,
+                                          uint32_t y, bool valid, uint32_t &t) {
+  t = 0;
+  if (!valid) return false;
+  const uint32_t yr = y >> 1;
+  uint32_t h = fib_hash(yr, shift);
+  while (true) {
+    const uint32_t x = T[h];
+    if (x == kEmpty) return false;
+    if ((x >> 1) == yr) {
+      t = x;
+      return true;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// LDS hash table of a source list: `cap` = next_pow2(4 d) slots (load
+// <= 1/4), slot = rank & (cap - 1), linear probing.  Out-lists are runs of
+// nearby ranks, so the low rank bits spread them with few collisions.
+__device__ __forceinline__ int table_log2(uint32_t d, int min_lg) {
+  int lg = clog2(4u * d);
+  return lg < min_lg ? min_lg : lg;
+}
+
+__device__ __forceinline__ bool hash_find_mask(const uint32_t *T, uint32_t mask, uint32_t y,
+                                               bool valid, uint32_t &t) {
+  t = 0;
+  if (!valid) return false;
+  const uint32_t yr = y >> 1;
+  uint32_t h = yr & mask;
+  while (true) {
+    const uint32_t x = T[h];
+    if (x == kEmpty) return false;
+    if ((x >> 1) == yr) {
+      t = x;
+      return true;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// One workgroup per source vertex (source lists longer than one wave).
 template <bool CLS>
 __global__ __launch_bounds__(256) void k_hash(DevCsr g, Ctrl *ctrl,
                                               const DispatchParams *P,
                                               const uint32_t *__restrict__ vlist) {
   extern __shared__ uint32_t T[];
   __shared__ uint8_t table[kBuckets];
-  __shared__ uint32_t s_item;
+  __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
-  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
   table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   const uint32_t len = ctrl->len[2];
   const uint32_t *__restrict__ col = g.col;
+  QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   while (true) {
     __syncthreads();
-    if (tid == 0) s_item = atomicAdd(&ctrl->work[2], 1u);
+    if (tid == 0) {
+      uint32_t f = 0, c = 0;
+      if (!queue_pop(ctrl->queue[0], len, 1u, cur, f, c)) c = 0;
+      s_first = f;
+      s_count = c;
+    }
     __syncthreads();
-    const uint32_t item = s_item;
-    if (item >= len) break;
-    const uint32_t a = vlist[len - 1 - item];  // highest ranks first
+    if (s_count == 0) break;
+    const uint32_t a = vlist[s_first];
     const uint32_t la = g.off[a], ea = g.off[a + 1], d = ea - la;
     const uint32_t span = span_of(col, la, ea);
-    int lg = clog2(2 * d);
-    if (lg < 6) lg = 6;
+    const int lg = table_log2(d, 6);
     const uint32_t cap = 1u << lg, mask = cap - 1u;
-    const int shift = 32 - lg;
     for (uint32_t k = tid; k < cap; k += blockDim.x) T[k] = kEmpty;
     __syncthreads();
     for (uint32_t k = tid; k < d; k += blockDim.x) {
       uint32_t x = col[la + k];
-      uint32_t h = fib_hash(x >> 1, shift);
+      uint32_t h = (x >> 1) & mask;
       while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
     }
     __syncthreads();
-    for (uint32_t j = la + wave; j < ea; j += nwaves) {
-      const uint32_t w = col[j];
-      const uint32_t b = w >> 1, sab = w & 1u;
-      const uint32_t pb = g.off[b], eb = g.off[b + 1];
-      if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) continue;
-      for (uint32_t base = pb; base < eb; base += 64) {
-        const bool valid = base + lane < eb;
-        const uint32_t y = valid ? col[base + lane] : kEmpty;
-        const uint32_t yr = y >> 1;
-        uint32_t h = fib_hash(yr, shift);
-        uint32_t t = kEmpty;
-        bool found = false;
-        if (valid) {
-          while (true) {
-            t = T[h];
-            if (t == kEmpty) break;
-            if ((t >> 1) == yr) {
-              found = true;
-              break;
-            }
-            h = (h + 1) & mask;
-          }
-        }
-        acc.add(found, sab + (y & 1u) + (t & 1u));
+    auto probe = [&](uint32_t y, bool valid, uint32_t &t) {
+      return hash_find_mask(T, mask, y, valid, t);
+    };
+    walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
+      if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
+      stream_list<CLS>(col, pb, eb, w & 1u, probe, acc);
+    });
+  }
+  acc.flush(ctrl);
+}
+
+// One wave per source vertex (source lists of at most kWaveMaxDeg entries):
+// each wave owns a kWaveCap-slot table.
+constexpr uint32_t kWaveCap = 256;
+constexpr uint32_t kWaveMaxDeg = kWaveCap / 4;
+constexpr uint32_t kWaveChunk = 8;  // vertices per queue pop
+
+template <bool CLS>
+__global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
+                                                   const DispatchParams *P,
+                                                   const uint32_t *__restrict__ vlist) {
+  __shared__ uint32_t Tall[4 * kWaveCap];
+  __shared__ uint8_t table[kBuckets];
+  __shared__ TuneParams tune;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t lane = __lane_id();
+  table[tid] = ctrl->table[tid];
+  if (tid == 0) tune = P->tune;
+  __syncthreads();
+  uint32_t *T = Tall + wave * kWaveCap;
+  const uint32_t len = ctrl->len[4];
+  const uint32_t *__restrict__ col = g.col;
+  QueueCursor cur{blockIdx.x & 7u, 0u};
+  Acc<CLS> acc;
+  while (true) {
+    uint32_t first = 0, count = 0;
+    if (lane == 0 && !queue_pop(ctrl->queue[2], len, kWaveChunk, cur, first, count)) count = 0;
+    first = __shfl(first, 0, 64);
+    count = __shfl(count, 0, 64);
+    if (count == 0) break;
+    for (uint32_t it = 0; it < count; it++) {
+      const uint32_t a = vlist[first + it];
+      const uint32_t la = g.off[a], ea = g.off[a + 1], d = ea - la;
+      const uint32_t span = span_of(col, la, ea);
+      const int lg = table_log2(d, 4);
+      const uint32_t cap = 1u << lg, mask = cap - 1u;
+      for (uint32_t k = lane; k < cap; k += 64) T[k] = kEmpty;
+      wave_lds_sync();
+      if (lane < d) {
+        uint32_t x = col[la + lane];
+        uint32_t h = (x >> 1) & mask;
+        while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
       }
+      wave_lds_sync();
+      auto probe = [&](uint32_t y, bool valid, uint32_t &t) {
+        return hash_find_mask(T, mask, y, valid, t);
+      };
+      walk_edges(g, la, ea, 1u, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
+        if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
+        stream_list<CLS>(col, pb, eb, w & 1u, probe, acc);
+      });
+      wave_lds_sync();
     }
   }
   acc.flush(ctrl);
@@ -480,22 +745,27 @@ __global__ __launch_bounds__(256) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                 const uint32_t *__restrict__ vlist) {
   extern __shared__ uint32_t B[];
   __shared__ uint8_t table[kBuckets];
-  __shared__ uint32_t s_item;
+  __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
-  const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
   table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   const uint32_t len = ctrl->len[3];
   const uint32_t *__restrict__ col = g.col;
+  QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   while (true) {
     __syncthreads();
-    if (tid == 0) s_item = atomicAdd(&ctrl->work[3], 1u);
+    if (tid == 0) {
+      uint32_t f = 0, c = 0;
+      if (!queue_pop(ctrl->queue[1], len, 1u, cur, f, c)) c = 0;
+      s_first = f;
+      s_count = c;
+    }
     __syncthreads();
-    const uint32_t item = s_item;
-    if (item >= len) break;
-    const uint32_t a = vlist[len - 1 - item];
+    if (s_count == 0) break;
+    const uint32_t a = vlist[s_first];
     const uint32_t la = g.off[a], ea = g.off[a + 1], d = ea - la;
     const uint32_t lo = col[la] >> 1;
     const uint32_t span = span_of(col, la, ea);
@@ -508,20 +778,17 @@ __global__ __launch_bounds__(256) void k_bitmap(DevCsr g, Ctrl *ctrl,
       atomicOr(&B[r >> 4], (1u | ((x & 1u) << 1)) << ((r & 15u) * 2u));
     }
     __syncthreads();
-    for (uint32_t j = la + wave; j < ea; j += nwaves) {
-      const uint32_t w = col[j];
-      const uint32_t b = w >> 1, sab = w & 1u;
-      const uint32_t pb = g.off[b], eb = g.off[b + 1];
-      if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_BITMAP) continue;
-      for (uint32_t base = pb; base < eb; base += 64) {
-        const bool valid = base + lane < eb;
-        const uint32_t y = valid ? col[base + lane] : kEmpty;
-        const uint32_t r = (y >> 1) - lo;  // wraps for y below the span
-        uint32_t bits = 0;
-        if (valid && r < span) bits = B[r >> 4] >> ((r & 15u) * 2u);
-        acc.add((bits & 1u) != 0u, sab + (y & 1u) + ((bits >> 1) & 1u));
-      }
-    }
+    auto probe = [&](uint32_t y, bool valid, uint32_t &t) {
+      const uint32_t r = (y >> 1) - lo;  // wraps for y below the span
+      uint32_t bits = 0;
+      if (valid && r < span) bits = B[r >> 4] >> ((r & 15u) * 2u);
+      t = (bits >> 1) & 1u;
+      return (bits & 1u) != 0u;
+    };
+    walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
+      if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_BITMAP) return;
+      stream_list<CLS>(col, pb, eb, w & 1u, probe, acc);
+    });
   }
   acc.flush(ctrl);
 }
@@ -542,7 +809,7 @@ inline int grid_for(uint64_t n, int threads, int cap) {
 }  // namespace
 
 size_t hash_lds_bytes(const TuneParams &t) {
-  return (size_t)4u << std::max(t.hash_max_log2 + 1, 6);
+  return (size_t)4u << std::max(t.hash_max_log2 + 2, 6);
 }
 size_t bitmap_lds_bytes(const TuneParams &t) {
   return (size_t)4u * ((1u << t.bitmap_span_log2) / 16u);
@@ -560,61 +827,91 @@ size_t bitmap_lds_bytes(const TuneParams &t) {
 // lists).
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
-                         int num_cus, hipStream_t st, hipEvent_t ev_bin_done) {
+                         int num_cus, hipStream_t st, const hipEvent_t *marks) {
   const uint64_t m = g.m;
   const uint32_t n = g.n;
   STC_TRY(cb.gcount.ensure(sizeof(Ctrl)));
-  STC_TRY(cb.lists.ensure(2 * (m ? m : 1) * 4));
+  const int bin_grid = grid_for(m, 256, num_cus * 8);
+  // per-workgroup segment size: the most edges one workgroup visits
+  const uint64_t stride = (uint64_t)bin_grid * 256;
+  const uint32_t seg = (uint32_t)(((m + stride - 1) / stride) * 256);
+  const uint64_t seg_words = (uint64_t)seg * bin_grid;
+  STC_TRY(cb.lists.ensure((2 * seg_words + 2 * (m ? m : 1)) * 4));
+  STC_TRY(cb.blk.ensure((4 * (uint64_t)bin_grid + 8) * 4));
   if (cb.vflag.bytes < ((uint64_t)n + 1) * 4) {
     STC_TRY(cb.vflag.ensure(((uint64_t)n + 1) * 4));
     hipLaunchKernelGGL(k_fill_words, dim3(grid_for((uint64_t)n + 1, 256, 8192)), dim3(256),
                        0, st, cb.vflag.as<uint32_t>(), cb.vflag.bytes / 4, 0u);
     STC_TRY(hipGetLastError());
   }
-  STC_TRY(cb.vlist.ensure(6 * ((uint64_t)n + 1) * 4));
+  STC_TRY(cb.vlist.ensure(9 * ((uint64_t)n + 1) * 4));
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
-  uint32_t *lane_list = cb.lists.as<uint32_t>();
+  uint32_t *lane_seg = cb.lists.as<uint32_t>();
+  uint32_t *wave_seg = lane_seg + seg_words;
+  uint32_t *lane_list = wave_seg + seg_words;
   uint32_t *wave_list = lane_list + (m ? m : 1);
+  uint32_t *blk_cnt = cb.blk.as<uint32_t>();
+  uint32_t *blk_pos = blk_cnt + 2 * bin_grid;
   uint32_t *vf = cb.vflag.as<uint32_t>();
   uint32_t *fh = cb.vlist.as<uint32_t>();
   uint32_t *ph = fh + (n + 1), *fb = ph + (n + 1), *pb = fb + (n + 1);
   uint32_t *vh = pb + (n + 1), *vb = vh + (n + 1);
+  uint32_t *fw = vb + (n + 1), *pw = fw + (n + 1), *vw = pw + (n + 1);
 
-  hipLaunchKernelGGL(k_fill_words, dim3(4), dim3(256), 0, st, (uint32_t *)ctrl,
+  if (marks) STC_TRY(hipEventRecord(marks[0], st));
+  hipLaunchKernelGGL(k_fill_words, dim3(8), dim3(256), 0, st, (uint32_t *)ctrl,
                      (uint64_t)(sizeof(Ctrl) / 4), 0u);
   STC_TRY(hipGetLastError());
-  const int bin_grid = grid_for(m, 256, num_cus * 8);
   hipLaunchKernelGGL(k_bin_hist, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp);
   STC_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_bin_scatter, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp,
-                     lane_list, wave_list, vf);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp, lane_seg,
+                     wave_seg, seg, blk_cnt, vf);
+  STC_TRY(hipGetLastError());
+  STC_TRY(scan_exclusive_u32(blk_cnt, blk_pos, 2 * (uint64_t)bin_grid, cb.scan, st));
+  hipLaunchKernelGGL(k_list_compact, dim3(2 * bin_grid), dim3(256), 0, st, lane_seg, wave_seg,
+                     seg, blk_cnt, blk_pos, (uint32_t)bin_grid, lane_list, wave_list, ctrl);
   STC_TRY(hipGetLastError());
   if (n > 0) {
     const int vgrid = grid_for(n, 256, num_cus * 8);
-    hipLaunchKernelGGL(k_vflag_split, dim3(vgrid), dim3(256), 0, st, vf, n, fh, fb);
+    hipLaunchKernelGGL(k_vflag_split, dim3(vgrid), dim3(256), 0, st, vf, g.off, n,
+                       kWaveMaxDeg, fh, fb, fw);
     STC_TRY(hipGetLastError());
     STC_TRY(scan_exclusive_u32(fh, ph, n, cb.scan, st));
     STC_TRY(scan_exclusive_u32(fb, pb, n, cb.scan, st));
-    hipLaunchKernelGGL(k_vlist_scatter, dim3(vgrid), dim3(256), 0, st, fh, ph, fb, pb, n,
-                       vh, vb, ctrl);
+    STC_TRY(scan_exclusive_u32(fw, pw, n, cb.scan, st));
+    hipLaunchKernelGGL(k_vlist_scatter, dim3(vgrid), dim3(256), 0, st, fh, ph, fb, pb, fw,
+                       pw, n, vh, vb, vw, ctrl);
     STC_TRY(hipGetLastError());
   }
-  if (ev_bin_done) STC_TRY(hipEventRecord(ev_bin_done, st));
+  if (marks) STC_TRY(hipEventRecord(marks[1], st));
   const int grid = num_cus * 8;
   const size_t hash_lds = hash_lds_bytes(hp.tune);
   const size_t bm_lds = bitmap_lds_bytes(hp.tune);
   const int hash_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (96 * 1024) / hash_lds));
   const int bm_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (128 * 1024) / bm_lds));
+  auto mark = [&](int i) -> hipError_t {
+    return marks ? hipEventRecord(marks[i], st) : hipSuccess;
+  };
   if (classes) {
     hipLaunchKernelGGL(k_merge_lane<true>, dim3(grid), dim3(256), 0, st, g, ctrl, lane_list);
+    STC_TRY(mark(2));
     hipLaunchKernelGGL(k_merge_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
+    STC_TRY(mark(3));
+    hipLaunchKernelGGL(k_hash_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
     hipLaunchKernelGGL(k_hash<true>, dim3(hash_grid), dim3(256), hash_lds, st, g, ctrl, dp, vh);
+    STC_TRY(mark(4));
     hipLaunchKernelGGL(k_bitmap<true>, dim3(bm_grid), dim3(256), bm_lds, st, g, ctrl, dp, vb);
+    STC_TRY(mark(5));
   } else {
     hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, lane_list);
+    STC_TRY(mark(2));
     hipLaunchKernelGGL(k_merge_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
+    STC_TRY(mark(3));
+    hipLaunchKernelGGL(k_hash_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
     hipLaunchKernelGGL(k_hash<false>, dim3(hash_grid), dim3(256), hash_lds, st, g, ctrl, dp, vh);
+    STC_TRY(mark(4));
     hipLaunchKernelGGL(k_bitmap<false>, dim3(bm_grid), dim3(256), bm_lds, st, g, ctrl, dp, vb);
+    STC_TRY(mark(5));
   }
   return hipGetLastError();
 }
@@ -625,7 +922,10 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_
   STC_TRY(hipStreamSynchronize(st));
   for (int k = 0; k < 4; k++) r.t[k] = h.t[k];
   (void)classes;
-  for (int k = 0; k < S_COUNT; k++) r.edges_by_strat[k] = h.strat_edges[k];
+  for (int k = 0; k < S_COUNT; k++) {
+    r.edges_by_strat[k] = h.strat_edges[k];
+    r.words_by_strat[k] = h.strat_words[k];
+  }
   for (int k = 0; k < kBuckets; k++) {
     r.table[k] = h.table[k];
     r.bucket_cnt[k] = h.bucket_cnt[k];

@@ -151,22 +151,26 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
                      std::string &err, hipStream_t st);
 
 struct CountBuffers {
-  DevBuf gcount, lists, vflag, vlist;
+  DevBuf gcount, lists, blk, vflag, vlist;
   ScanScratch scan;
 };
 
 struct CountResult {
   uint64_t t[4] = {0, 0, 0, 0};  // t0..t3 when classes, else t[0]=bal t[1]=unbal
   uint64_t edges_by_strat[S_COUNT] = {0, 0, 0, 0, 0};
+  uint64_t words_by_strat[S_COUNT] = {0, 0, 0, 0, 0};
   uint8_t table[kBuckets];
   uint32_t bucket_cnt[kBuckets];
 };
 
 // Enqueue binning + counting on `st` (no host sync inside).  hp is the host
-// copy of *dp (launch shapes come from its tune part).
+// copy of *dp (launch shapes come from its tune part).  marks (nullable, 6
+// events) are recorded before binning, after binning and after each of the
+// merge-lane, merge-wave, hash and bitmap kernels.
+constexpr int kMarks = 6;
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
-                         int num_cus, hipStream_t st, hipEvent_t ev_bin_done);
+                         int num_cus, hipStream_t st, const hipEvent_t *marks);
 // Read back totals and per-strategy sizes after the stream completes.
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r,
                       hipStream_t st);

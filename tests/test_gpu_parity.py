@@ -154,8 +154,18 @@ def test_classes_vs_oracle(ctx):
         assert (b, un) == (want["t3"] + want["t1"], want["t2"] + want["t0"])
 
 
+def _hip():
+    """The HIP runtime libstc.so is linked against (already loaded)."""
+    import ctypes as C
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    return h
+
+
 def test_repeat_and_device_input(ctx):
-    import torch
+    import ctypes as C
     g = stc.gen_workload(1)
     want = oracle_counts(g.n, g.u, g.v, g.sign)
     say("repeat")
@@ -163,11 +173,18 @@ def test_repeat_and_device_input(ctx):
     st = ctx.stats()
     assert st["ms_count"] > 0 and st["m_oriented"] == 100000
     say("device input")
-    du = torch.from_numpy(g.u.astype(np.int32)).cuda()
-    dv = torch.from_numpy(g.v.astype(np.int32)).cuda()
-    ds = torch.from_numpy(g.sign.copy()).cuda()
-    torch.cuda.synchronize()
-    got = ctx.count_device(g.n, g.m, du.data_ptr(), dv.data_ptr(), ds.data_ptr())
+    hip = _hip()
+    ptrs = []
+    for arr in (g.u, g.v, g.sign):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), arr.nbytes) == 0
+        assert hip.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) == 0  # host -> device
+        ptrs.append(p.value)
+    try:
+        got = ctx.count_device(g.n, g.m, *ptrs)
+    finally:
+        for p in ptrs:
+            hip.hipFree(p)
     assert got == want
 
 
