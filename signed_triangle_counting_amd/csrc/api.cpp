@@ -61,7 +61,7 @@ int hip_fail(stc_ctx *c, hipError_t e, const char *where) {
 void default_tune(TuneParams &t) {
   for (int i = 0; i < kBucketSide; i++) t.merge_wave_min_log2[i] = 5;
   t.hash_max_log2 = 10;
-  t.bitmap_span_log2 = 17;
+  t.bitmap_span_log2 = 18;
   t.hash_block = 256;
   t.fixed_skew_log2 = 2;
   t.fixed_hub_log2 = 9;
@@ -103,7 +103,7 @@ bool load_tune(const std::string &path, TuneParams &t) {
     } else if (k == "hash_max_log2") {
       if (!(is >> x.hash_max_log2) || x.hash_max_log2 < 4 || x.hash_max_log2 > 13) return false;
     } else if (k == "bitmap_span_log2") {
-      if (!(is >> x.bitmap_span_log2) || x.bitmap_span_log2 < 8 || x.bitmap_span_log2 > 18)
+      if (!(is >> x.bitmap_span_log2) || x.bitmap_span_log2 < 10 || x.bitmap_span_log2 > 19)
         return false;
     } else if (k == "fixed_skew_log2") {
       if (!(is >> x.fixed_skew_log2)) return false;

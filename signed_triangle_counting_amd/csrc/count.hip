@@ -28,6 +28,12 @@
 namespace stc {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// Source lists of at most kWaveMaxDeg entries are hashed by one wave; the
+// bitmap kernel takes source lists of kWaveMaxDeg+1 .. kBitmapMaxDeg entries.
+constexpr uint32_t kWaveMaxDeg = 64;
+constexpr uint32_t kBitmapMaxDeg = 2048;
+constexpr int kVertexBlock = 256;   // threads per workgroup of the hash kernel
+constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kernel
 
 // Device control block, cleared at the start of every counting run.
 struct Ctrl {
@@ -133,7 +139,9 @@ __device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table,
   if (da_eff == 0 || db == 0) return S_SKIP;
   int bk = bucket_of(da_eff, db);
   uint8_t s = table[bk];
-  if (s == S_BITMAP && span_a > (1u << t.bitmap_span_log2)) s = S_HASH;
+  if (s == S_BITMAP && (span_a > (1u << t.bitmap_span_log2) || dfull_a <= kWaveMaxDeg ||
+                        dfull_a > kBitmapMaxDeg))
+    s = S_HASH;
   if (s == S_HASH && dfull_a > (1u << t.hash_max_log2)) s = merge_variant(bk, t);
   return s;
 }
@@ -386,6 +394,7 @@ __global__ void k_vlist_scatter(const uint32_t *__restrict__ fh,
 
 // ---- class accumulation -----------------------------------------------------
 // neg = number of negative edges of a found triangle (0..3): class T(3-neg).
+// In two-class mode c[0] counts balanced (even neg) and c[1] unbalanced.
 template <bool CLS>
 struct Acc {
   uint64_t c[4] = {0, 0, 0, 0};
@@ -401,11 +410,49 @@ struct Acc {
       c[1] += __popcll(__ballot(found && (neg & 1)));   // unbalanced
     }
   }
+  // Wave-uniform totals of one edge (u,v) with sign bit sab: e[k] triangles
+  // whose other two edges carry k negatives (two-class mode: e[0] even,
+  // e[1] odd).
+  __device__ __forceinline__ void add_edge(uint32_t sab, uint32_t e0, uint32_t e1, uint32_t e2) {
+    if (CLS) {
+      // negatives = k + sab -> class T(3 - k - sab)
+      if (sab) {
+        c[2] += e0;
+        c[1] += e1;
+        c[0] += e2;
+      } else {
+        c[3] += e0;
+        c[2] += e1;
+        c[1] += e2;
+      }
+    } else {
+      c[0] += sab ? e1 : e0;
+      c[1] += sab ? e0 : e1;
+    }
+  }
   __device__ __forceinline__ void flush(Ctrl *ctrl) {
 #pragma unroll
     for (int k = 0; k < 4; k++) wave_add(&ctrl->t[k], c[k]);
   }
 };
+
+// Per-lane probe result folded into wave-uniform edge totals.  q: bit 0 =
+// found, bit 1 = neg(u,x); nb = neg(v,x).
+template <bool CLS>
+__device__ __forceinline__ void tally(uint32_t q, uint32_t nb, uint32_t &e0, uint32_t &e1,
+                                      uint32_t &e2) {
+  if (CLS) {
+    const uint32_t n2 = (q >> 1) + nb;
+    const bool f = (q & 1u) != 0u;
+    e0 += (uint32_t)__popcll(__ballot(f && n2 == 0u));
+    e1 += (uint32_t)__popcll(__ballot(f && n2 == 1u));
+    e2 += (uint32_t)__popcll(__ballot(f && n2 == 2u));
+  } else {
+    const uint32_t v = q ^ (nb << 1);  // 1: found, even; 3: found, odd
+    e0 += (uint32_t)__popcll(__ballot(v == 1u));
+    e1 += (uint32_t)__popcll(__ballot(v == 3u));
+  }
+}
 
 // ---- merge: one lane per edge ----------------------------------------------
 template <bool CLS>
@@ -521,10 +568,6 @@ __global__ __launch_bounds__(256) void k_merge_wave(DevCsr g, Ctrl *ctrl,
 }
 
 // ---- vertex-grouped kernels -------------------------------------------------
-__device__ __forceinline__ uint32_t fib_hash(uint32_t r, int shift) {
-  return (r * 0x9E3779B1u) >> shift;
-}
-
 // Wave-scope LDS ordering for the wave-per-vertex variant (no workgroup
 // barrier is needed when one wave owns an LDS region).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -535,29 +578,72 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 constexpr int kUnroll = 4;  // independent 64-element loads in flight per wave
 
-// Stream N+(b) = col[pb, eb) through `probe` (one element per lane, kUnroll
-// chunks of 64 loaded before any is probed) and accumulate the classes.
-template <bool CLS, class Probe>
-__device__ __forceinline__ void stream_list(const uint32_t *__restrict__ col, uint32_t pb,
-                                            uint32_t eb, uint32_t sab, const Probe &probe,
-                                            Acc<CLS> &acc) {
+// Stream N+(v) = col[pb, eb) against the LDS 2-bit bitmap of N+(u) (entry r =
+// bits 2r, 2r+1 for rank lo + r) and add the edge's triangles to acc.  With
+// d2 = y - 2 lo = 2 (rank - lo) + neg(v,x), the word is d2 >> 5, the shift is
+// d2 & 30 and the span test is d2 < 2 span; padding (kEmpty) always fails it.
+template <bool CLS>
+__device__ __forceinline__ void stream_bitmap(const uint32_t *__restrict__ col, uint32_t pb,
+                                              uint32_t eb, uint32_t sab,
+                                              const uint32_t *__restrict__ B, uint32_t lo2,
+                                              uint32_t span2, Acc<CLS> &acc) {
   const uint32_t lane = __lane_id();
+  uint32_t e0 = 0, e1 = 0, e2 = 0;
   for (uint32_t base = pb; base < eb; base += 64u * kUnroll) {
     uint32_t y[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
-      uint32_t idx = base + u * 64u + lane;
+      const uint32_t idx = base + u * 64u + lane;
       y[u] = idx < eb ? col[idx] : kEmpty;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       if (base + u * 64u >= eb) break;  // wave-uniform
-      const bool valid = base + u * 64u + lane < eb;
-      uint32_t t = 0;
-      const bool found = probe(y[u], valid, t);
-      acc.add(found, sab + (y[u] & 1u) + (t & 1u));
+      const uint32_t d2 = y[u] - lo2;
+      uint32_t w = 0;
+      if (d2 < span2) w = B[d2 >> 5];
+      tally<CLS>((w >> (d2 & 30u)) & 3u, d2 & 1u, e0, e1, e2);
     }
   }
+  acc.add_edge(sab, e0, e1, e2);
+}
+
+// Same against an LDS hash table of N+(u) (slot = rank & mask, linear probe).
+template <bool CLS>
+__device__ __forceinline__ void stream_hash(const uint32_t *__restrict__ col, uint32_t pb,
+                                            uint32_t eb, uint32_t sab,
+                                            const uint32_t *__restrict__ T, uint32_t mask,
+                                            Acc<CLS> &acc) {
+  const uint32_t lane = __lane_id();
+  uint32_t e0 = 0, e1 = 0, e2 = 0;
+  for (uint32_t base = pb; base < eb; base += 64u * kUnroll) {
+    uint32_t y[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const uint32_t idx = base + u * 64u + lane;
+      y[u] = idx < eb ? col[idx] : kEmpty;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      if (base + u * 64u >= eb) break;  // wave-uniform
+      const uint32_t yr = y[u] >> 1;
+      uint32_t q = 0;
+      if (y[u] != kEmpty) {
+        uint32_t h = yr & mask;
+        while (true) {
+          const uint32_t x = T[h];
+          if (x == kEmpty) break;
+          if ((x >> 1) == yr) {
+            q = 1u | ((x & 1u) << 1);
+            break;
+          }
+          h = (h + 1) & mask;
+        }
+      }
+      tally<CLS>(q, y[u] & 1u, e0, e1, e2);
+    }
+  }
+  acc.add_edge(sab, e0, e1, e2);
 }
 
 // Walk the out-edges j = first, first+step, ... of source a (ending at ea),
@@ -565,44 +651,26 @@ __device__ __forceinline__ void stream_list(const uint32_t *__restrict__ col, ui
 template <class Body>
 __device__ __forceinline__ void walk_edges(const DevCsr &g, uint32_t first, uint32_t ea,
                                            uint32_t step, const Body &body) {
-  uint32_t j = first;
+  uint32_t j = __builtin_amdgcn_readfirstlane(first);
   uint32_t w = 0, pb = 0, eb = 0;
   if (j < ea) {
-    w = g.col[j];
-    pb = g.off[w >> 1];
-    eb = g.off[(w >> 1) + 1];
+    w = __builtin_amdgcn_readfirstlane(g.col[j]);
+    pb = __builtin_amdgcn_readfirstlane(g.off[w >> 1]);
+    eb = __builtin_amdgcn_readfirstlane(g.off[(w >> 1) + 1]);
   }
   while (j < ea) {
     const uint32_t jn = j + step;
     uint32_t wn = 0, pbn = 0, ebn = 0;
     if (jn < ea) {
-      wn = g.col[jn];
-      pbn = g.off[wn >> 1];
-      ebn = g.off[(wn >> 1) + 1];
+      wn = __builtin_amdgcn_readfirstlane(g.col[jn]);
+      pbn = __builtin_amdgcn_readfirstlane(g.off[wn >> 1]);
+      ebn = __builtin_amdgcn_readfirstlane(g.off[(wn >> 1) + 1]);
     }
     body(j, w, pb, eb);
     j = jn;
     w = wn;
     pb = pbn;
     eb = ebn;
-  }
-}
-
-// Linear-probe lookup of rank yr in an LDS table of `mask + 1` slots.
-__device__ __forceinline__ bool hash_find(const uint32_t *T, uint32_t mask, int shift,
-                                          uint32_t y, bool valid, uint32_t &t) {
-  t = 0;
-  if (!valid) return false;
-  const uint32_t yr = y >> 1;
-  uint32_t h = fib_hash(yr, shift);
-  while (true) {
-    const uint32_t x = T[h];
-    if (x == kEmpty) return false;
-    if ((x >> 1) == yr) {
-      t = x;
-      return true;
-    }
-    h = (h + 1) & mask;
   }
 }
 
@@ -614,35 +682,18 @@ __device__ __forceinline__ int table_log2(uint32_t d, int min_lg) {
   return lg < min_lg ? min_lg : lg;
 }
 
-__device__ __forceinline__ bool hash_find_mask(const uint32_t *T, uint32_t mask, uint32_t y,
-                                               bool valid, uint32_t &t) {
-  t = 0;
-  if (!valid) return false;
-  const uint32_t yr = y >> 1;
-  uint32_t h = yr & mask;
-  while (true) {
-    const uint32_t x = T[h];
-    if (x == kEmpty) return false;
-    if ((x >> 1) == yr) {
-      t = x;
-      return true;
-    }
-    h = (h + 1) & mask;
-  }
-}
-
 // One workgroup per source vertex (source lists longer than one wave).
 template <bool CLS>
-__global__ __launch_bounds__(256) void k_hash(DevCsr g, Ctrl *ctrl,
+__global__ __launch_bounds__(kVertexBlock) void k_hash(DevCsr g, Ctrl *ctrl,
                                               const DispatchParams *P,
                                               const uint32_t *__restrict__ vlist) {
   extern __shared__ uint32_t T[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nwaves = blockDim.x >> 6;
-  table[tid] = ctrl->table[tid];
+  if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   const uint32_t len = ctrl->len[2];
   const uint32_t *__restrict__ col = g.col;
@@ -658,8 +709,9 @@ __global__ __launch_bounds__(256) void k_hash(DevCsr g, Ctrl *ctrl,
     }
     __syncthreads();
     if (s_count == 0) break;
-    const uint32_t a = vlist[s_first];
-    const uint32_t la = g.off[a], ea = g.off[a + 1], d = ea - la;
+    const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[__builtin_amdgcn_readfirstlane(s_first)]);
+    const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
+    const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
     const uint32_t span = span_of(col, la, ea);
     const int lg = table_log2(d, 6);
     const uint32_t cap = 1u << lg, mask = cap - 1u;
@@ -671,12 +723,9 @@ __global__ __launch_bounds__(256) void k_hash(DevCsr g, Ctrl *ctrl,
       while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
     }
     __syncthreads();
-    auto probe = [&](uint32_t y, bool valid, uint32_t &t) {
-      return hash_find_mask(T, mask, y, valid, t);
-    };
     walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
       if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
-      stream_list<CLS>(col, pb, eb, w & 1u, probe, acc);
+      stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
     });
   }
   acc.flush(ctrl);
@@ -684,8 +733,7 @@ __global__ __launch_bounds__(256) void k_hash(DevCsr g, Ctrl *ctrl,
 
 // One wave per source vertex (source lists of at most kWaveMaxDeg entries):
 // each wave owns a kWaveCap-slot table.
-constexpr uint32_t kWaveCap = 256;
-constexpr uint32_t kWaveMaxDeg = kWaveCap / 4;
+constexpr uint32_t kWaveCap = 4 * kWaveMaxDeg;
 constexpr uint32_t kWaveChunk = 8;  // vertices per queue pop
 
 template <bool CLS>
@@ -725,12 +773,9 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
         while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
       }
       wave_lds_sync();
-      auto probe = [&](uint32_t y, bool valid, uint32_t &t) {
-        return hash_find_mask(T, mask, y, valid, t);
-      };
       walk_edges(g, la, ea, 1u, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
         if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
-        stream_list<CLS>(col, pb, eb, w & 1u, probe, acc);
+        stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
       });
       wave_lds_sync();
     }
@@ -738,19 +783,24 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   acc.flush(ctrl);
 }
 
-// ---- vertex-grouped: LDS 2-bit bitmap over N+(u)'s rank span --------------
+// ---- vertex-grouped: LDS 2-bit bitmap over N+(u)'s rank span ---------------
+// Entry r (bits 2r, 2r+1 of the bitmap) holds (present, negative) for rank
+// lo + r, so a probe is one LDS read.  Words touched by a vertex are cleared
+// after it, so the bitmap is zeroed only once per workgroup.
 template <bool CLS>
-__global__ __launch_bounds__(256) void k_bitmap(DevCsr g, Ctrl *ctrl,
-                                                const DispatchParams *P,
-                                                const uint32_t *__restrict__ vlist) {
+__global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
+                                                         const DispatchParams *P,
+                                                         const uint32_t *__restrict__ vlist,
+                                                         uint32_t bitmap_words) {
   extern __shared__ uint32_t B[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nwaves = blockDim.x >> 6;
-  table[tid] = ctrl->table[tid];
+  if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
+  for (uint32_t k = tid; k < bitmap_words; k += blockDim.x) B[k] = 0u;
   const uint32_t len = ctrl->len[3];
   const uint32_t *__restrict__ col = g.col;
   QueueCursor cur{blockIdx.x & 7u, 0u};
@@ -765,30 +815,23 @@ __global__ __launch_bounds__(256) void k_bitmap(DevCsr g, Ctrl *ctrl,
     }
     __syncthreads();
     if (s_count == 0) break;
-    const uint32_t a = vlist[s_first];
-    const uint32_t la = g.off[a], ea = g.off[a + 1], d = ea - la;
-    const uint32_t lo = col[la] >> 1;
+    const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[__builtin_amdgcn_readfirstlane(s_first)]);
+    const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
+    const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
     const uint32_t span = span_of(col, la, ea);
-    const uint32_t words = (span + 15) >> 4;
-    for (uint32_t k = tid; k < words; k += blockDim.x) B[k] = 0u;
-    __syncthreads();
     for (uint32_t k = tid; k < d; k += blockDim.x) {
-      uint32_t x = col[la + k];
-      uint32_t r = (x >> 1) - lo;
+      const uint32_t x = col[la + k];
+      const uint32_t r = (x >> 1) - lo;
       atomicOr(&B[r >> 4], (1u | ((x & 1u) << 1)) << ((r & 15u) * 2u));
     }
     __syncthreads();
-    auto probe = [&](uint32_t y, bool valid, uint32_t &t) {
-      const uint32_t r = (y >> 1) - lo;  // wraps for y below the span
-      uint32_t bits = 0;
-      if (valid && r < span) bits = B[r >> 4] >> ((r & 15u) * 2u);
-      t = (bits >> 1) & 1u;
-      return (bits & 1u) != 0u;
-    };
     walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
       if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_BITMAP) return;
-      stream_list<CLS>(col, pb, eb, w & 1u, probe, acc);
+      stream_bitmap<CLS>(col, pb, eb, w & 1u, B, lo << 1, span << 1, acc);
     });
+    __syncthreads();
+    for (uint32_t k = tid; k < d; k += blockDim.x) B[((col[la + k] >> 1) - lo) >> 4] = 0u;
   }
   acc.flush(ctrl);
 }
@@ -812,7 +855,7 @@ size_t hash_lds_bytes(const TuneParams &t) {
   return (size_t)4u << std::max(t.hash_max_log2 + 2, 6);
 }
 size_t bitmap_lds_bytes(const TuneParams &t) {
-  return (size_t)4u * ((1u << t.bitmap_span_log2) / 16u);
+  return (size_t)((1u << t.bitmap_span_log2) / 4u);
 }
 
 #define STC_TRY(expr)                \
@@ -887,8 +930,21 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
   const int grid = num_cus * 8;
   const size_t hash_lds = hash_lds_bytes(hp.tune);
   const size_t bm_lds = bitmap_lds_bytes(hp.tune);
-  const int hash_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (96 * 1024) / hash_lds));
-  const int bm_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (128 * 1024) / bm_lds));
+  const int hash_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (128 * 1024) / hash_lds));
+  const int bm_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (144 * 1024) / bm_lds));
+  const uint32_t bm_words = (1u << hp.tune.bitmap_span_log2) / 16u;
+  static bool attr_done = false;
+  if (!attr_done) {
+    STC_TRY(hipFuncSetAttribute((const void *)k_bitmap<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    STC_TRY(hipFuncSetAttribute((const void *)k_bitmap<false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    STC_TRY(hipFuncSetAttribute((const void *)k_hash<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    STC_TRY(hipFuncSetAttribute((const void *)k_hash<false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    attr_done = true;
+  }
   auto mark = [&](int i) -> hipError_t {
     return marks ? hipEventRecord(marks[i], st) : hipSuccess;
   };
@@ -898,9 +954,11 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL(k_merge_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
     STC_TRY(mark(3));
     hipLaunchKernelGGL(k_hash_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
-    hipLaunchKernelGGL(k_hash<true>, dim3(hash_grid), dim3(256), hash_lds, st, g, ctrl, dp, vh);
+    hipLaunchKernelGGL(k_hash<true>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
+                       dp, vh);
     STC_TRY(mark(4));
-    hipLaunchKernelGGL(k_bitmap<true>, dim3(bm_grid), dim3(256), bm_lds, st, g, ctrl, dp, vb);
+    hipLaunchKernelGGL(k_bitmap<true>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
+                       dp, vb, bm_words);
     STC_TRY(mark(5));
   } else {
     hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, lane_list);
@@ -908,9 +966,11 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL(k_merge_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
     STC_TRY(mark(3));
     hipLaunchKernelGGL(k_hash_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
-    hipLaunchKernelGGL(k_hash<false>, dim3(hash_grid), dim3(256), hash_lds, st, g, ctrl, dp, vh);
+    hipLaunchKernelGGL(k_hash<false>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
+                       dp, vh);
     STC_TRY(mark(4));
-    hipLaunchKernelGGL(k_bitmap<false>, dim3(bm_grid), dim3(256), bm_lds, st, g, ctrl, dp, vb);
+    hipLaunchKernelGGL(k_bitmap<false>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
+                       dp, vb, bm_words);
     STC_TRY(mark(5));
   }
   return hipGetLastError();
