@@ -1,0 +1,62 @@
+/*
+ * stc_train.h -- dispatcher training and tuning hooks of the signed triangle
+ * counter.  No counterpart in the reference: its only dispatch is a fixed
+ * 4x length-skew rule (Signed_TC/GPU Implementations/GlobalMEM/
+ * signed_tc.hxx:48,65).  These calls let the counter time its own strategies
+ * per edge bucket, train the dispatcher MLP on those timings with a HIP
+ * kernel, and apply autotuned launch parameters.  A table or model can only
+ * change which kernel counts an edge, never the count.
+ *
+ * Buckets: oriented edge (u->v) falls in bucket 16*a + b with
+ * a = min(15, ceil(log2(#out-neighbours of u ranked above v))),
+ * b = min(15, ceil(log2 d+(v))); edges with either part empty cannot close a
+ * triangle and are skipped.
+ */
+#ifndef STC_TRAIN_H
+#define STC_TRAIN_H
+
+#include <stdint.h>
+
+#include "stc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { STC_TABLE_SKIP = 0, STC_TABLE_MERGE = 1, STC_TABLE_HASH = 2, STC_TABLE_BITMAP = 3 };
+#define STC_BUCKETS 256
+#define STC_MLP_FEATURES 6
+
+/* Oriented edges per bucket of the prepared graph (runs the binning pass). */
+int stc_bucket_histogram(stc_ctx *ctx, uint32_t out[STC_BUCKETS]);
+
+/* Count the prepared graph with an explicit strategy per bucket
+ * (STC_TABLE_*); buckets set to STC_TABLE_SKIP are not counted, so the result
+ * is partial unless every non-empty bucket has a strategy. */
+int stc_count_with_table(stc_ctx *ctx, const uint8_t table[STC_BUCKETS], int repeat,
+                         stc_counts *out);
+
+/* Dispatcher features of bucket `bucket` holding `count` edges in a graph of
+ * n vertices and m oriented edges (the same vector the device computes). */
+void stc_bucket_features(int bucket, uint32_t count, uint32_t n, uint64_t m,
+                         float out[STC_MLP_FEATURES]);
+
+/* Train the dispatcher MLP (6 -> 16 ReLU -> 3) on the device by full-batch
+ * Adam over nsamp samples: feat (nsamp x 6) and cost (nsamp x 3, time of the
+ * merge / hash / bitmap strategy for that sample, any unit).  The loss is the
+ * softmax-expected normalised cost weighted by each sample's time at stake.
+ * Deterministic for a given seed.  Writes the model file (text, checksummed)
+ * to out_path and loads it into the context. */
+int stc_train_dispatcher(stc_ctx *ctx, const float *feat, const float *cost, int nsamp,
+                         int iters, float lr, uint64_t seed, const char *out_path);
+
+/* Reload "dispatch_mlp.txt" / "tune.txt" from dir (NULL: the shipped ones). */
+int stc_load_models(stc_ctx *ctx, const char *dir);
+
+/* Apply tune parameters given in the tune.txt text format. */
+int stc_set_tune(stc_ctx *ctx, const char *text);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STC_TRAIN_H */

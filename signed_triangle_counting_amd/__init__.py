@@ -41,7 +41,11 @@ EXPORTED = [
     "stc_load_graph", "stc_parse_graph", "stc_write_graph", "stc_graph_free",
     "stc_gen_er", "stc_gen_grid", "stc_gen_rmat", "stc_gen_hub", "stc_gen_batch",
     "stc_batch_free", "stc_gen_workload", "stc_gen_workload_batch",
+    "stc_bucket_histogram", "stc_count_with_table", "stc_bucket_features",
+    "stc_train_dispatcher", "stc_load_models", "stc_set_tune",
 ]
+
+TABLE = {"skip": 0, "merge": 1, "hash": 2, "bitmap": 3}
 
 
 class StcError(RuntimeError):
@@ -147,6 +151,14 @@ def lib():
                                 C.c_double, C.c_double, C.c_uint64, C.c_int, P(P(_HostBatch))]
     L.stc_gen_workload.argtypes = [C.c_int, C.c_int, P(P(_HostGraph))]
     L.stc_gen_workload_batch.argtypes = [C.c_uint32, C.c_int, P(P(_HostBatch))]
+    L.stc_bucket_histogram.argtypes = [C.c_void_p, C.c_void_p]
+    L.stc_count_with_table.argtypes = [C.c_void_p, C.c_void_p, C.c_int, P(Counts)]
+    L.stc_bucket_features.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p]
+    L.stc_bucket_features.restype = None
+    L.stc_train_dispatcher.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                       C.c_float, C.c_uint64, C.c_char_p]
+    L.stc_load_models.argtypes = [C.c_void_p, C.c_char_p]
+    L.stc_set_tune.argtypes = [C.c_void_p, C.c_char_p]
     _lib_handle = L
     return L
 
@@ -373,6 +385,34 @@ class Counter:
         _check(lib().stc_get_stats(self._h, C.byref(s)), self._h)
         return s.as_dict()
 
+    # ---- dispatcher training / tuning hooks (include/stc_train.h) ----
+    def bucket_histogram(self):
+        out = np.zeros(256, np.uint32)
+        _check(lib().stc_bucket_histogram(self._h, out.ctypes.data), self._h)
+        return out
+
+    def count_with_table(self, table, repeat=1):
+        t = np.ascontiguousarray(table, np.uint8)
+        assert t.shape == (256,)
+        out = Counts()
+        _check(lib().stc_count_with_table(self._h, t.ctypes.data, repeat, C.byref(out)), self._h)
+        return int(out.balanced), int(out.unbalanced)
+
+    def train_dispatcher(self, feat, cost, out_path, iters=3000, lr=0.01, seed=1):
+        f = np.ascontiguousarray(feat, np.float32)
+        c = np.ascontiguousarray(cost, np.float32)
+        assert f.shape[1] == 6 and c.shape[1] == 3 and f.shape[0] == c.shape[0]
+        _check(lib().stc_train_dispatcher(self._h, f.ctypes.data, c.ctypes.data, f.shape[0],
+                                          iters, lr, seed, out_path.encode()), self._h)
+
+    def load_models(self, model_dir=None):
+        rc = lib().stc_load_models(self._h, model_dir.encode() if model_dir else None)
+        self.model_status = rc
+        return rc
+
+    def set_tune(self, text):
+        _check(lib().stc_set_tune(self._h, text.encode()), self._h)
+
     def count_batch(self, batch, dispatch="learned", repeat=1):
         """Per-graph (balanced, unbalanced) for a Batch or a list of Graphs."""
         if isinstance(batch, Batch):
@@ -386,6 +426,12 @@ class Counter:
                                      C.byref(self._opts(dispatch, repeat, False)), out),
                self._h)
         return [(int(out[i].balanced), int(out[i].unbalanced)) for i in range(k)]
+
+
+def bucket_features(bucket, count, n, m):
+    out = np.zeros(6, np.float32)
+    lib().stc_bucket_features(bucket, count, n, m, out.ctypes.data)
+    return out
 
 
 class signed_tc:

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/stc.h"
+#include "../../include/stc_train.h"
 #include "internal.h"
 
 using namespace stc;
@@ -87,9 +88,7 @@ uint64_t fnv1a(const std::string &s) {
 }
 
 // Tune table: "stc-tune v1" then "key values..." lines.
-bool load_tune(const std::string &path, TuneParams &t) {
-  std::ifstream f(path);
-  if (!f) return false;
+bool parse_tune(std::istream &f, TuneParams &t) {
   std::string line;
   if (!std::getline(f, line) || line.rfind("stc-tune v1", 0) != 0) return false;
   TuneParams x = t;
@@ -99,9 +98,11 @@ bool load_tune(const std::string &path, TuneParams &t) {
     if (!(is >> k) || k[0] == '#') continue;
     if (k == "merge_wave_min_log2") {
       for (int i = 0; i < kBucketSide; i++)
-        if (!(is >> x.merge_wave_min_log2[i])) return false;
+        if (!(is >> x.merge_wave_min_log2[i]) || x.merge_wave_min_log2[i] < 0 ||
+            x.merge_wave_min_log2[i] > 16)
+          return false;
     } else if (k == "hash_max_log2") {
-      if (!(is >> x.hash_max_log2) || x.hash_max_log2 < 4 || x.hash_max_log2 > 13) return false;
+      if (!(is >> x.hash_max_log2) || x.hash_max_log2 < 4 || x.hash_max_log2 > 12) return false;
     } else if (k == "bitmap_span_log2") {
       if (!(is >> x.bitmap_span_log2) || x.bitmap_span_log2 < 10 || x.bitmap_span_log2 > 19)
         return false;
@@ -113,6 +114,12 @@ bool load_tune(const std::string &path, TuneParams &t) {
   }
   t = x;
   return true;
+}
+
+bool load_tune(const std::string &path, TuneParams &t) {
+  std::ifstream f(path);
+  if (!f) return false;
+  return parse_tune(f, t);
 }
 
 // Dispatcher model: "stc-dispatch-mlp v1", dims line, arrays by name, then
@@ -176,6 +183,36 @@ bool load_model(const std::string &path, MlpModel &m, std::string &why) {
   }
   m = x;
   return true;
+}
+
+std::string fmt_arr(const char *name, const float *v, int n) {
+  std::string out = name;
+  char buf[32];
+  for (int i = 0; i < n; i++) {
+    std::snprintf(buf, sizeof buf, " %.9g", (double)v[i]);
+    out += buf;
+  }
+  out += '\n';
+  return out;
+}
+
+bool write_model(const std::string &path, const MlpModel &m, const std::string &note) {
+  std::string body = "stc-dispatch-mlp v1\n";
+  body += "dims 6 16 3\n";
+  body += fmt_arr("mean", m.mean, kMlpIn);
+  body += fmt_arr("inv_std", m.inv_std, kMlpIn);
+  body += fmt_arr("w1", &m.w1[0][0], kMlpHidden * kMlpIn);
+  body += fmt_arr("b1", m.b1, kMlpHidden);
+  body += fmt_arr("w2", &m.w2[0][0], kMlpOut * kMlpHidden);
+  body += fmt_arr("b2", m.b2, kMlpOut);
+  body += "arch gfx950\n";
+  body += "note " + note + "\n";
+  char chk[64];
+  std::snprintf(chk, sizeof chk, "check %016llx\n", (unsigned long long)fnv1a(body));
+  std::ofstream f(path, std::ios::binary | std::ios::trunc);
+  if (!f) return false;
+  f << body << chk;
+  return (bool)f;
 }
 
 float median(std::vector<float> v) {
@@ -312,16 +349,14 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   return STC_OK;
 }
 
-int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
-  if (!c || !out) return STC_E_ARG;
+}  // extern "C"
+
+static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bool classes,
+                      stc_counts *out, CountResult *res_out) {
   if (!c->prepared) return fail(c, STC_E_ARG, "no prepared graph");
   (void)hipSetDevice(c->device);
-  int dispatch = opts ? opts->dispatch : STC_LEARNED;
-  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
-  bool classes = opts && opts->classes;
-  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
-    return fail(c, STC_E_ARG, "bad dispatch");
-  c->hp.mode = dispatch;
+  c->hp.mode = mode;
+  if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   c->hp.graph_feat[0] = (float)std::log2((double)c->g.n + 1.0);
   c->hp.graph_feat[1] = (float)std::log2((double)c->g.m + 1.0);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
@@ -369,9 +404,21 @@ int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
     c->stats.ms_kernel[k] = median(kt[k]);
     c->stats.bytes_kernel[k] = 4.0 * (double)res.words_by_strat[k + 1];
   }
-  c->stats.dispatch_used =
-      (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
+  c->stats.dispatch_used = (mode == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : mode;
+  if (res_out) *res_out = res;
   return STC_OK;
+}
+
+extern "C" {
+
+int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
+  if (!c || !out) return STC_E_ARG;
+  int dispatch = opts ? opts->dispatch : STC_LEARNED;
+  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
+  bool classes = opts && opts->classes;
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+    return fail(c, STC_E_ARG, "bad dispatch");
+  return count_core(c, dispatch, nullptr, repeat, classes, out, nullptr);
 }
 
 int stc_count(stc_ctx *c, const stc_edges *g, const stc_opts *opts, stc_counts *out) {
@@ -421,5 +468,128 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
     int rc = stc_count(c, &graphs[i], opts, &out[i]);
     if (rc) return rc;
   }
+  return STC_OK;
+}
+
+// ---- training / tuning hooks (include/stc_train.h) -------------------------
+extern "C" int stc_bucket_histogram(stc_ctx *c, uint32_t out[STC_BUCKETS]) {
+  if (!c || !out) return STC_E_ARG;
+  uint8_t skip[kBuckets];
+  std::memset(skip, 0, sizeof skip);
+  stc_counts cnt{0, 0};
+  CountResult res;
+  int rc = count_core(c, D_TABLE, skip, 1, false, &cnt, &res);
+  if (rc) return rc;
+  for (int k = 0; k < kBuckets; k++) out[k] = res.bucket_cnt[k];
+  return STC_OK;
+}
+
+extern "C" int stc_count_with_table(stc_ctx *c, const uint8_t table[STC_BUCKETS], int repeat,
+                                    stc_counts *out) {
+  if (!c || !table || !out) return STC_E_ARG;
+  for (int k = 0; k < kBuckets; k++)
+    if (table[k] > STC_TABLE_BITMAP) return fail(c, STC_E_ARG, "bad table entry");
+  return count_core(c, D_TABLE, table, repeat > 0 ? repeat : 1, false, out, nullptr);
+}
+
+extern "C" void stc_bucket_features(int bucket, uint32_t count, uint32_t n, uint64_t m,
+                                    float out[STC_MLP_FEATURES]) {
+  float la = (float)(bucket / kBucketSide), lb = (float)(bucket % kBucketSide);
+  out[0] = la;
+  out[1] = lb;
+  out[2] = la - lb;
+  out[3] = std::log2((float)count + 1.0f);
+  out[4] = (float)std::log2((double)n + 1.0);
+  out[5] = (float)std::log2((double)m + 1.0);
+}
+
+extern "C" int stc_train_dispatcher(stc_ctx *c, const float *feat, const float *cost,
+                                    int nsamp, int iters, float lr, uint64_t seed,
+                                    const char *out_path) {
+  if (!c || !feat || !cost || nsamp <= 0 || iters <= 0 || !out_path) return STC_E_ARG;
+  (void)hipSetDevice(c->device);
+  MlpModel m{};
+  // feature normalisation
+  for (int k = 0; k < kMlpIn; k++) {
+    double mu = 0, var = 0;
+    for (int i = 0; i < nsamp; i++) mu += feat[i * kMlpIn + k];
+    mu /= nsamp;
+    for (int i = 0; i < nsamp; i++) {
+      double d = feat[i * kMlpIn + k] - mu;
+      var += d * d;
+    }
+    double sd = std::sqrt(var / nsamp);
+    m.mean[k] = (float)mu;
+    m.inv_std[k] = (float)(1.0 / (sd > 1e-3 ? sd : 1e-3));
+  }
+  std::vector<float> x((size_t)nsamp * kMlpIn), r((size_t)nsamp * kMlpOut), w(nsamp);
+  double wsum = 0;
+  for (int i = 0; i < nsamp; i++) {
+    for (int k = 0; k < kMlpIn; k++)
+      x[(size_t)i * kMlpIn + k] = (feat[i * kMlpIn + k] - m.mean[k]) * m.inv_std[k];
+    float lo = cost[i * kMlpOut], hi = lo;
+    for (int s = 1; s < kMlpOut; s++) {
+      lo = std::min(lo, cost[i * kMlpOut + s]);
+      hi = std::max(hi, cost[i * kMlpOut + s]);
+    }
+    for (int s = 0; s < kMlpOut; s++)
+      r[(size_t)i * kMlpOut + s] = (cost[i * kMlpOut + s] - lo) / (hi - lo + 1e-9f);
+    w[i] = hi - lo;
+    wsum += w[i];
+  }
+  for (int i = 0; i < nsamp; i++) w[i] = (float)(w[i] / (wsum > 0 ? wsum : 1.0));
+  // deterministic initialisation: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in))
+  const int np = mlp_param_count();
+  std::vector<float> params(np, 0.f);
+  uint64_t st = seed ^ 0x6d6c70ull;
+  auto uni = [&](float a) {
+    st += 0x9e3779b97f4a7c15ULL;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    return a * (2.0f * (float)((double)(z >> 11) * 0x1.0p-53) - 1.0f);
+  };
+  const int nw1 = kMlpHidden * kMlpIn, nw2 = kMlpOut * kMlpHidden;
+  for (int i = 0; i < nw1; i++) params[i] = uni(1.0f / std::sqrt((float)kMlpIn));
+  for (int i = 0; i < nw2; i++) params[nw1 + kMlpHidden + i] = uni(1.0f / std::sqrt((float)kMlpHidden));
+  hipError_t e = train_mlp_device(x.data(), r.data(), w.data(), nsamp, iters, lr, params.data(), c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "train");
+  mlp_unpack(params.data(), m);
+  char note[160];
+  std::snprintf(note, sizeof note, "samples %d iters %d lr %g seed %llu", nsamp, iters, (double)lr,
+                (unsigned long long)seed);
+  if (!write_model(out_path, m, note)) return fail(c, STC_E_IO, "cannot write model");
+  c->hp.model = m;
+  c->hp.have_model = 1;
+  c->model_status = STC_OK;
+  return STC_OK;
+}
+
+extern "C" int stc_load_models(stc_ctx *c, const char *dir) {
+  if (!c) return STC_E_ARG;
+  std::string d = dir ? std::string(dir) : lib_dir() + "/models";
+  TuneParams t;
+  default_tune(t);
+  load_tune(d + "/tune.txt", t);
+  c->hp.tune = t;
+  std::string why;
+  if (load_model(d + "/dispatch_mlp.txt", c->hp.model, why)) {
+    c->hp.have_model = 1;
+    c->model_status = STC_OK;
+    return STC_OK;
+  }
+  c->hp.have_model = 0;
+  c->model_status = STC_E_MODEL;
+  c->err = "dispatcher model unavailable (" + why + "); fixed rule in use";
+  return STC_E_MODEL;
+}
+
+extern "C" int stc_set_tune(stc_ctx *c, const char *text) {
+  if (!c || !text) return STC_E_ARG;
+  std::istringstream is(text);
+  TuneParams t = c->hp.tune;
+  if (!parse_tune(is, t)) return fail(c, STC_E_ARG, "bad tune text");
+  c->hp.tune = t;
   return STC_OK;
 }

@@ -119,8 +119,13 @@ __device__ uint8_t decide(int bk, uint32_t cnt, const DispatchParams &p) {
       return S_HASH;
     case D_FORCE_BITMAP:
       return S_BITMAP;
-    case D_TABLE:
-      return p.host_table[bk];
+    case D_TABLE:  // host table: 0 skip, 1 merge, 2 hash, 3 bitmap
+      switch (p.host_table[bk]) {
+        case 1: return merge_variant(bk, p.tune);
+        case 2: return S_HASH;
+        case 3: return S_BITMAP;
+        default: return S_SKIP;
+      }
     case D_LEARNED:
       if (p.have_model) return mlp_rule(bk, cnt, p);
       return fixed_rule(bk, p.tune);

@@ -184,10 +184,13 @@ hipError_t enqueue_batch_persistent(const DevCsr &g,
                                     uint32_t k, uint64_t *per_graph,
                                     int num_cus, hipStream_t st);
 
-// MLP training on device (full-batch gradient descent, one workgroup,
-// fixed reduction order -> deterministic).
-hipError_t train_mlp_device(const float *feat, const float *cost, int nsamp,
-                            int iters, float lr, float wd, MlpModel *model_io,
-                            hipStream_t st);
+// MLP training on device (full-batch Adam in one wave64, fixed reduction
+// order -> deterministic).  x: nsamp x kMlpIn normalised features; r: nsamp x
+// kMlpOut costs normalised to [0, 1] per sample; w: per-sample weights;
+// params_io: flat parameters (w1, b1, w2, b2), updated in place.
+hipError_t train_mlp_device(const float *x, const float *r, const float *w, int nsamp,
+                            int iters, float lr, float *params_io, hipStream_t st);
+int mlp_param_count();
+void mlp_unpack(const float *params, MlpModel &m);
 
 }  // namespace stc

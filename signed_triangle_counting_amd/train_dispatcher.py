@@ -1,0 +1,166 @@
+"""Train the learned dispatcher from timings the counter generates itself.
+
+    python -m signed_triangle_counting_amd.train_dispatcher --out DIR [--tune-dir DIR]
+
+For each graph of a synthetic training set (seeds distinct from the benchmark
+workloads), the counter's own binning pass gives the oriented-edge histogram
+over the 256 buckets.  One coordinate-descent pass over the buckets then
+times the whole counting phase with each bucket under each strategy (merge,
+LDS hash, hub bitmap) while every other bucket keeps its current best choice;
+those three in-context times are the bucket's costs.  The samples (bucket
+features, three costs) train the 6-16-3 MLP with the device training kernel
+(stc_train_dispatcher).  The model can only change which kernel counts an
+edge: the evaluation here checks that every dispatch mode returns the same
+counts.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import signed_triangle_counting_amd as stc  # noqa: E402
+
+STRATS = ("merge", "hash", "bitmap")
+np.seterr(all="ignore")
+
+
+def training_graphs(quick=False):
+    """(name, generator thunk) pairs; no benchmark seed is reused."""
+    g = [
+        ("rmat16_ef16", lambda: stc.gen_rmat(16, 16, 0.57, 0.19, 0.19, 0.25, 101)),
+        ("rmat18_ef16", lambda: stc.gen_rmat(18, 16, 0.57, 0.19, 0.19, 0.25, 102)),
+        ("rmat20_ef8", lambda: stc.gen_rmat(20, 8, 0.57, 0.19, 0.19, 0.25, 103)),
+        ("rmat20_ef16", lambda: stc.gen_rmat(20, 16, 0.57, 0.19, 0.19, 0.25, 104)),
+        ("rmat19_ef32", lambda: stc.gen_rmat(19, 32, 0.55, 0.2, 0.2, 0.25, 105)),
+        ("rmat21_ef16", lambda: stc.gen_rmat(21, 16, 0.57, 0.19, 0.19, 0.25, 106)),
+        ("er_1e5_1e6", lambda: stc.gen_er(100000, 1000000, 0.3, 107)),
+        ("er_1e6_6e6", lambda: stc.gen_er(1000000, 6000000, 0.3, 108)),
+        ("er_2e4_2e6", lambda: stc.gen_er(20000, 2000000, 0.3, 109)),
+        ("grid_500", lambda: stc.gen_grid(500, 500, 0.5, 0.2, 110)),
+        ("grid_300x1000", lambda: stc.gen_grid(300, 1000, 0.9, 0.2, 111)),
+        ("hub_2m", lambda: stc.gen_hub(2000000, 2100000, 8, 1.2, 0.6, 20000, 0.1, 112)),
+        ("hub_6m", lambda: stc.gen_hub(6000000, 6400000, 12, 1.0, 0.5, 60000, 0.1, 113)),
+        ("hub_4m", lambda: stc.gen_hub(4000000, 4300000, 4, 1.5, 0.7, 40000, 0.1, 114)),
+        ("rmat22_ef16", lambda: stc.gen_rmat(22, 16, 0.57, 0.19, 0.19, 0.25, 115)),
+        ("er_2e6_2e7", lambda: stc.gen_er(2000000, 20000000, 0.3, 116)),
+    ]
+    return g[:4] + g[6:7] + g[11:12] if quick else g
+
+
+def time_table(ctx, table, reps):
+    ctx.count_with_table(table, repeat=1)
+    ctx.count_with_table(table, repeat=reps)
+    return float(ctx.stats()["ms_count"])
+
+
+def collect(ctx, name, g, reps=3, log=print):
+    """In-context costs by one coordinate-descent pass over the buckets.
+
+    Start from the best single-strategy table; visit buckets heaviest first
+    (edges x 2^(a+b)); for each, time the whole counting phase with that
+    bucket under each strategy and every other bucket at its current choice,
+    keep the fastest, and record the three times as the bucket's costs.
+    Per-vertex staging is shared by a vertex's buckets, so costs measured in
+    context, not bucket by bucket in isolation, are what the choice changes.
+    """
+    ctx.prepare(g)
+    st = ctx.stats()
+    n, m = st["n"], st["m_oriented"]
+    hist = ctx.bucket_histogram()
+    nz = [int(b) for b in np.nonzero(hist)[0]]
+    table = np.zeros(256, np.uint8)
+    best_t, best_s = None, None
+    for s in STRATS:
+        t = np.zeros(256, np.uint8)
+        t[nz] = stc.TABLE[s]
+        ms = time_table(ctx, t, reps)
+        if best_t is None or ms < best_t:
+            best_t, best_s = ms, s
+    table[nz] = stc.TABLE[best_s]
+    order = sorted(nz, key=lambda b: -float(hist[b]) * 2.0 ** ((b // 16) + (b % 16)))
+    feats, costs = [], []
+    for b in order:
+        c = []
+        for s in STRATS:
+            t = table.copy()
+            t[b] = stc.TABLE[s]
+            c.append(time_table(ctx, t, reps))
+        table[b] = stc.TABLE[STRATS[int(np.argmin(c))]]
+        feats.append(stc.bucket_features(b, int(hist[b]), n, m))
+        costs.append(c)
+    final = time_table(ctx, table, reps)
+    log("  %-14s n=%d m+=%d buckets=%d start=%s %.3fms -> tuned table %.3fms" % (
+        name, n, m, len(nz), best_s, best_t, final))
+    return np.array(feats, np.float32), np.array(costs, np.float32), order, final
+
+
+def evaluate(ctx, graphs, log=print):
+    rows = []
+    for name, thunk in graphs:
+        g = thunk()
+        ctx.prepare(g)
+        res = {}
+        for mode in ("fixed", "learned", "merge", "hash", "bitmap"):
+            ctx.count_prepared(dispatch=mode, repeat=2)
+            cnt = ctx.count_prepared(dispatch=mode, repeat=5)
+            res[mode] = (ctx.stats()["ms_count"], cnt)
+        counts = {v[1] for v in res.values()}
+        if len(counts) != 1:
+            raise RuntimeError("dispatch changed a count on %s: %r" % (name, res))
+        row = {k: round(v[0], 4) for k, v in res.items()}
+        row["graph"] = name
+        row["learned_vs_fixed"] = round(res["fixed"][0] / res["learned"][0], 3)
+        rows.append(row)
+        log("  %-14s fixed %.3f learned %.3f merge %.3f hash %.3f bitmap %.3f  (x%.2f)" % (
+            name, row["fixed"], row["learned"], row["merge"], row["hash"], row["bitmap"],
+            row["learned_vs_fixed"]))
+    return rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True, help="directory for dispatch_mlp.txt")
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--iters", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=0.01)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--tune-dir", default=None, help="tune.txt to apply first")
+    args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+    ctx = stc.Counter(0, model_dir=args.tune_dir)
+    t0 = time.time()
+    feats, costs, meta = [], [], []
+    print("collecting per-bucket strategy timings", flush=True)
+    graphs = training_graphs(args.quick)
+    tuned = {}
+    for name, thunk in graphs:
+        f, c, b, final = collect(ctx, name, thunk(), log=lambda s: print(s, flush=True))
+        feats.append(f)
+        costs.append(c)
+        meta += [(name, x) for x in b]
+        tuned[name] = round(final, 4)
+    F = np.concatenate(feats)
+    Cst = np.concatenate(costs)
+    np.savez(os.path.join(args.out, "dispatch_samples.npz"), feat=F, cost=Cst)
+    print("samples %d collected in %.1fs" % (len(F), time.time() - t0), flush=True)
+    path = os.path.join(args.out, "dispatch_mlp.txt")
+    ctx.train_dispatcher(F, Cst, path, iters=args.iters, lr=args.lr, seed=args.seed)
+    print("trained ->", path, flush=True)
+    # agreement with the per-sample best strategy (training set)
+    ctx.load_models(args.out)
+    rows = evaluate(ctx, graphs, log=lambda s: print(s, flush=True))
+    for r in rows:
+        r["coordinate_descent_table"] = tuned.get(r["graph"])
+    with open(os.path.join(args.out, "dispatch_eval.json"), "w") as f:
+        json.dump({"samples": int(len(F)), "graphs": rows}, f, indent=1)
+    print("done in %.1fs" % (time.time() - t0), flush=True)
+
+
+if __name__ == "__main__":
+    main()

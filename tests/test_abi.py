@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     names = set()
-    for h in ("stc.h", "stc_gen.h"):
+    for h in ("stc.h", "stc_gen.h", "stc_train.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for mm in re.finditer(r"^[A-Za-z_][\w \t\*]*?\b(stc_\w+)\s*\(", txt, flags=re.M):
