@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-KERNELS = ["merge_lane", "merge_wave", "hash", "bitmap"]
+KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap"]
 WORKLOADS = {1: "er_10k_100k", 2: "grid_960x960_diag", 3: "rmat22_ef16", 4: "hub_18m"}
 
 
@@ -44,6 +44,8 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hub", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--traffic-from", default=None,
+                    help="write profiles/traffic.json from a rocprofv3 PMC directory and exit")
     return ap.parse_args()
 
 
@@ -70,27 +72,80 @@ def traffic_for(kernel, workload):
 
 def cpu_baseline(g, budget_s):
     """The oracle's merge count (tc_merge_sign_enu.cpp:131-145 restated) over
-    every stride-th source vertex, stride chosen so the sample takes about
-    budget_s seconds; returns oriented edges per second."""
+    every stride-th source vertex; a first pass over 1/64 of the vertices
+    sets the stride so the timed sample takes about budget_s seconds (the
+    whole graph when that fits).  Returns oriented edges per second."""
     import oracle
     threads = min(os.cpu_count() or 1, 16)
     t0 = time.time()
     csr = oracle.Csr(g.n, g.u, g.v, g.sign, threads)
     build_s = time.time() - t0
-    probe = csr.merge_sample(1024, 0, threads)
-    per_vertex_s = probe["seconds"] / max(1, (g.n + 1023) // 1024)
-    est_full = per_vertex_s * g.n
-    stride = max(1, int(est_full / budget_s) + 1)
-    s = csr.merge_sample(stride, 1 % stride, threads)
+    probe = csr.merge_sample(64, 7, threads)
+    est_full = probe["seconds"] * 64.0
+    stride = max(1, int(round(est_full / budget_s)))
+    phase = 1 % stride
+    s = csr.merge_sample(stride, phase, threads)
     return {
         "value": s["edges"] / s["seconds"] if s["seconds"] > 0 else None,
         "unit": "oriented edges/s",
         "cores": threads,
         "kind": "port",
-        "sample": "oracle merge count over source vertices v %% %d == %d (%d of %d oriented "
-                  "edges, %.1f s); oracle CSR build %.1f s not included"
-                  % (stride, 1 % stride, s["edges"], csr.m, s["seconds"], build_s),
+        "sample": "oracle merge count (OpenMP, %d threads) over source vertices v %% %d == %d: "
+                  "%d of %d oriented edges in %.1f s; oracle CSR build (%.1f s) not included"
+                  % (threads, stride, phase, s["edges"], csr.m, s["seconds"], build_s),
     }
+
+
+def traffic_from(pmc_dir, workload, out_path):
+    """Per-launch HBM traffic of each kernel from rocprofv3 PMC runs:
+    pmc_dir/{fetch,write,copy_fetch,copy_write}/run_counter_collection.csv.
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are each scaled by what the
+    same counter reads for a device copy of known size (copy_*: 1 GiB)."""
+    import csv
+    import collections
+
+    def read(sub):
+        agg = collections.defaultdict(lambda: collections.defaultdict(list))
+        with open(os.path.join(pmc_dir, sub, "run_counter_collection.csv")) as f:
+            for row in csv.DictReader(f):
+                name = row["Kernel_Name"]
+                agg[name][row["Counter_Name"]].append(float(row["Counter_Value"]))
+        return agg
+
+    def copy_factor(sub, counter):
+        for name, d in read(sub).items():
+            if "copyBuffer" in name and counter in d:
+                v = sorted(d[counter])[len(d[counter]) // 2]
+                return (1 << 20) / v  # 1 GiB = 2^20 KiB moved each way
+        return None
+
+    ff = copy_factor("copy_fetch", "FETCH_SIZE")
+    fw = copy_factor("copy_write", "WRITE_SIZE")
+    fetch, write = read("fetch"), read("write")
+    names = {"k_merge_lane": "merge_lane", "k_merge_wave": "merge_wave",
+             "k_hash_wave": "hash_wave", "k_hash<": "hash", "k_bitmap": "bitmap"}
+    out = {}
+    try:
+        with open(out_path) as f:
+            out = json.load(f)
+    except (OSError, ValueError):
+        pass
+    rec = {}
+    for name, d in fetch.items():
+        key = next((v for k, v in names.items() if k in name), None)
+        if key is None or "FETCH_SIZE" not in d:
+            continue
+        fk = sum(d["FETCH_SIZE"]) / len(d["FETCH_SIZE"])
+        wk = 0.0
+        for wn, wd in write.items():
+            if wn == name and "WRITE_SIZE" in wd:
+                wk = sum(wd["WRITE_SIZE"]) / len(wd["WRITE_SIZE"])
+        rec[key] = {"bytes_per_launch": round(1024.0 * (fk * ff + wk * fw)),
+                    "fetch_kib_raw": fk, "write_kib_raw": wk}
+    out[workload] = dict(rec, _calibration={"fetch_factor": ff, "write_factor": fw})
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print(json.dumps(out[workload], indent=1))
 
 
 def hub_learned_vs_fixed(ctx, stc, steps):
@@ -109,6 +164,10 @@ def hub_learned_vs_fixed(ctx, stc, steps):
 
 def main():
     args = parse_args()
+    if args.traffic_from:
+        traffic_from(args.traffic_from, WORKLOADS[args.workload],
+                     os.path.join(ROOT, "profiles", "traffic.json"))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build counts one graph (or one batch) per "
               "GPU and has no multi-GPU path; use --gpus 1", file=sys.stderr)
@@ -134,7 +193,7 @@ def main():
     gold = goldens().get("workloads", {}).get(str(args.workload))
     exact = None if gold is None else (counts == (gold["balanced"], gold["unbalanced"]))
 
-    k = max(range(4), key=lambda i: st["ms_kernel"][i])
+    k = max(range(len(KERNELS)), key=lambda i: st["ms_kernel"][i])
     kname = KERNELS[k]
     kms = st["ms_kernel"][k]
     achieved = st["bytes_kernel"][k] / (kms * 1e-3) / 1e9 if kms > 0 else 0.0

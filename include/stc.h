@@ -101,9 +101,10 @@ typedef struct stc_stats {
   uint64_t edges_bitmap;
   uint64_t edges_skipped;  /* edges that cannot close a triangle */
   int dispatch_used;       /* stc_dispatch actually applied */
-  float ms_kernel[4];      /* merge-lane, merge-wave, hash, bitmap kernel time
-                              (hipEvent on the library stream), median */
-  double bytes_kernel[4];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
+  float ms_kernel[5];      /* merge-lane, merge-wave, wave-hash, workgroup-hash
+                              and bitmap kernel time (hipEvent on the library
+                              stream), median over repeats */
+  double bytes_kernel[5];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
                               oriented edges each of those kernels counted */
 } stc_stats;
 

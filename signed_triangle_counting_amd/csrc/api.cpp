@@ -361,7 +361,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   c->hp.graph_feat[1] = (float)std::log2((double)c->g.m + 1.0);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   if (e != hipSuccess) return hip_fail(c, e, "params upload");
-  std::vector<float> times, bins, kt[4];
+  std::vector<float> times, bins, kt[kKernels];
   for (int r = 0; r < repeat; r++) {
     e = hipEventRecord(c->ev[0], c->st);
     if (e == hipSuccess)
@@ -371,7 +371,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
     float ms = 0.f, msb = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
     if (e == hipSuccess) e = hipEventElapsedTime(&msb, c->marks[0], c->marks[1]);
-    for (int k = 0; k < 4 && e == hipSuccess; k++) {
+    for (int k = 0; k < kKernels && e == hipSuccess; k++) {
       float t = 0.f;
       e = hipEventElapsedTime(&t, c->marks[k + 1], c->marks[k + 2]);
       kt[k].push_back(t);
@@ -400,9 +400,14 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   c->stats.edges_merge = res.edges_by_strat[S_MERGE_LANE] + res.edges_by_strat[S_MERGE_WAVE];
   c->stats.edges_hash = res.edges_by_strat[S_HASH];
   c->stats.edges_bitmap = res.edges_by_strat[S_BITMAP];
-  for (int k = 0; k < 4; k++) {
+  // kernel order: merge-lane, merge-wave, wave-hash, workgroup-hash, bitmap
+  const uint64_t words_k[kKernels] = {
+      res.words_by_strat[S_MERGE_LANE], res.words_by_strat[S_MERGE_WAVE],
+      res.words_by_strat[S_COUNT], res.words_by_strat[S_HASH] - res.words_by_strat[S_COUNT],
+      res.words_by_strat[S_BITMAP]};
+  for (int k = 0; k < kKernels; k++) {
     c->stats.ms_kernel[k] = median(kt[k]);
-    c->stats.bytes_kernel[k] = 4.0 * (double)res.words_by_strat[k + 1];
+    c->stats.bytes_kernel[k] = 4.0 * (double)words_k[k];
   }
   c->stats.dispatch_used = (mode == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : mode;
   if (res_out) *res_out = res;

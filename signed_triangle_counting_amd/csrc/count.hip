@@ -46,8 +46,9 @@ struct Ctrl {
   uint32_t queue[4][8][32];          // XCD work queues: one counter per XCD,
                                      // 128 bytes apart
   unsigned long long t[4];           // class counters
-  unsigned long long strat_edges[8]; // oriented edges per strategy
-  unsigned long long strat_words[8]; // sum of d+(u) + d+(v) per strategy
+  unsigned long long strat_edges[8]; // oriented edges per strategy (slot 5:
+                                     // hash edges of wave-hashed sources)
+  unsigned long long strat_words[8]; // sum of d+(u) + d+(v), same slots
   uint8_t table[kBuckets];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl alignment");
@@ -282,15 +283,15 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
     s_len[0] = s_len[1] = 0;
   }
   __syncthreads();
-  uint32_t cnt_s[S_COUNT] = {0, 0, 0, 0, 0};
-  uint64_t words_s[S_COUNT] = {0, 0, 0, 0, 0};
+  uint32_t cnt_s[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
+  uint64_t words_s[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
   uint32_t *lane_out = lane_seg + (uint64_t)blockIdx.x * seg;
   uint32_t *wave_out = wave_seg + (uint64_t)blockIdx.x * seg;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
     uint8_t s = S_SKIP;
-    uint32_t a = 0, words = 0;
+    uint32_t a = 0, words = 0, db_of_words = 0;
     const bool valid = e < g.m;
     if (valid) {
       a = g.src[e];
@@ -299,6 +300,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
       uint32_t da = ea - (uint32_t)e - 1u;
       uint32_t db = g.off[b + 1] - g.off[b];
       words = (ea - la) + db;
+      db_of_words = db;
       if (da && db) {
         uint8_t s0 = table[bucket_of(da, db)];
         uint32_t span = s0 == S_BITMAP ? span_of(g.col, la, ea) : 0u;
@@ -319,9 +321,12 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
       cnt_s[k] += hit;
       words_s[k] += hit ? words : 0u;
     }
+    const bool wave_hashed = valid && s == S_HASH && (words - db_of_words) <= kWaveMaxDeg;
+    cnt_s[S_COUNT] += wave_hashed;
+    words_s[S_COUNT] += wave_hashed ? words : 0u;
   }
 #pragma unroll
-  for (int k = 0; k < S_COUNT; k++) {
+  for (int k = 0; k <= S_COUNT; k++) {
     uint64_t c = cnt_s[k], w = words_s[k];
     for (int d = 32; d > 0; d >>= 1) {
       c += __shfl_down(c, d, 64);
@@ -959,24 +964,26 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL(k_merge_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
     STC_TRY(mark(3));
     hipLaunchKernelGGL(k_hash_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
+    STC_TRY(mark(4));
     hipLaunchKernelGGL(k_hash<true>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
                        dp, vh);
-    STC_TRY(mark(4));
+    STC_TRY(mark(5));
     hipLaunchKernelGGL(k_bitmap<true>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
                        dp, vb, bm_words);
-    STC_TRY(mark(5));
+    STC_TRY(mark(6));
   } else {
     hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, lane_list);
     STC_TRY(mark(2));
     hipLaunchKernelGGL(k_merge_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
     STC_TRY(mark(3));
     hipLaunchKernelGGL(k_hash_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
+    STC_TRY(mark(4));
     hipLaunchKernelGGL(k_hash<false>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
                        dp, vh);
-    STC_TRY(mark(4));
+    STC_TRY(mark(5));
     hipLaunchKernelGGL(k_bitmap<false>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
                        dp, vb, bm_words);
-    STC_TRY(mark(5));
+    STC_TRY(mark(6));
   }
   return hipGetLastError();
 }
@@ -987,7 +994,7 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_
   STC_TRY(hipStreamSynchronize(st));
   for (int k = 0; k < 4; k++) r.t[k] = h.t[k];
   (void)classes;
-  for (int k = 0; k < S_COUNT; k++) {
+  for (int k = 0; k <= S_COUNT; k++) {
     r.edges_by_strat[k] = h.strat_edges[k];
     r.words_by_strat[k] = h.strat_words[k];
   }

@@ -157,17 +157,19 @@ struct CountBuffers {
 
 struct CountResult {
   uint64_t t[4] = {0, 0, 0, 0};  // t0..t3 when classes, else t[0]=bal t[1]=unbal
-  uint64_t edges_by_strat[S_COUNT] = {0, 0, 0, 0, 0};
-  uint64_t words_by_strat[S_COUNT] = {0, 0, 0, 0, 0};
+  // slots S_SKIP..S_BITMAP, then S_COUNT: hash edges of wave-hashed sources
+  uint64_t edges_by_strat[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
+  uint64_t words_by_strat[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
   uint8_t table[kBuckets];
   uint32_t bucket_cnt[kBuckets];
 };
 
 // Enqueue binning + counting on `st` (no host sync inside).  hp is the host
-// copy of *dp (launch shapes come from its tune part).  marks (nullable, 6
+// copy of *dp (launch shapes come from its tune part).  marks (nullable, 7
 // events) are recorded before binning, after binning and after each of the
-// merge-lane, merge-wave, hash and bitmap kernels.
-constexpr int kMarks = 6;
+// merge-lane, merge-wave, wave-hash, workgroup-hash and bitmap kernels.
+constexpr int kMarks = 7;
+constexpr int kKernels = 5;
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
                          int num_cus, hipStream_t st, const hipEvent_t *marks);
