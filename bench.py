@@ -43,6 +43,7 @@ def parse_args():
     ap.add_argument("--dispatch", default="learned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hub", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--traffic-from", default=None,
                     help="write profiles/traffic.json from a rocprofv3 PMC directory and exit")
@@ -162,6 +163,23 @@ def hub_learned_vs_fixed(ctx, stc, steps):
     return out
 
 
+def batch_workload(ctx, stc, steps):
+    """4,096 independent graphs counted in one persistent launch per step,
+    replayed from a captured hipGraph; per-graph counts against the fixture."""
+    b = stc.gen_workload_batch(4096)
+    t0 = time.perf_counter()
+    got = ctx.count_batch(b, dispatch="learned", repeat=steps)
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    per = goldens().get("batch", {}).get("per_graph")
+    exact = None if per is None else all(list(got[i]) == per[i][1:3] for i in range(b.k))
+    return {"graphs": b.k, "lines": b.total_m, "m_oriented": st["m_oriented"],
+            "ms_per_replay": round(st["ms_count"], 4),
+            "oriented_edges_per_s": st["m_oriented"] / (st["ms_count"] * 1e-3),
+            "build_ms": round(st["ms_build"], 3), "upload_ms": round(st["ms_upload"], 3),
+            "call_wall_s": round(wall, 3), "exact_vs_reference": exact}
+
+
 def main():
     args = parse_args()
     if args.traffic_from:
@@ -225,6 +243,8 @@ def main():
     }
     if not args.no_hub:
         line["learned_vs_fixed"] = hub_learned_vs_fixed(ctx, stc, max(3, args.steps // 2))
+    if not args.no_batch:
+        line["batch"] = batch_workload(ctx, stc, 3)
     ctx.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)

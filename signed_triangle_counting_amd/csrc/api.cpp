@@ -255,6 +255,7 @@ int stc_create(int device, const char *model_dir, stc_ctx **out) {
   for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
   for (int i = 0; i < kMarks && e == hipSuccess; i++) e = hipEventCreate(&c->marks[i]);
   if (e == hipSuccess) e = hipMalloc(&c->dp, sizeof(DispatchParams));
+  if (e == hipSuccess) e = init_kernel_attributes();
   if (e != hipSuccess) {
     int rc = hip_fail(c, e, "stc_create");
     stc_destroy(c);
@@ -466,13 +467,165 @@ float stc_run(stc_ctx *c, const stc_edges *g, uint64_t *total_balanced,
 
 }  // extern "C"
 
+// Batched counting (reference: each graph would be one run of
+// tc_merge_sign_enu.cpp:46-154).  The k graphs become one disjoint union:
+// line ids are shifted into [vbase[g], vbase[g+1]) on the device, one CSR is
+// built with ranks grouped per graph, and the counting phase -- binning plus
+// ONE persistent kernel that adds each item's triangles to its graph's
+// counters -- is captured once in a hipGraph and replayed.  The first
+// (eager) run allocates every buffer, so the captured sequence allocates
+// nothing; each replay must reproduce the eager per-graph counts.
 extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
                                const stc_opts *opts, stc_counts *out) {
   if (!c || (!graphs && k) || (!out && k)) return STC_E_ARG;
+  if (k == 0) return STC_OK;
+  (void)hipSetDevice(c->device);
+  int dispatch = opts ? opts->dispatch : STC_LEARNED;
+  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+    return fail(c, STC_E_ARG, "bad dispatch");
+  if (opts && opts->classes) return fail(c, STC_E_ARG, "batched counting reports two classes");
+  std::vector<uint32_t> vbase(k + 1, 0);
+  std::vector<uint64_t> ebase(k + 1, 0);
+  bool contiguous = true;
   for (uint32_t i = 0; i < k; i++) {
-    int rc = stc_count(c, &graphs[i], opts, &out[i]);
+    int rc = check_graph(c, &graphs[i]);
     if (rc) return rc;
+    uint64_t vb = (uint64_t)vbase[i] + graphs[i].n;
+    if (vb >= (1ull << 31)) return fail(c, STC_E_TOO_LARGE, "batch has >= 2^31 vertices");
+    vbase[i + 1] = (uint32_t)vb;
+    ebase[i + 1] = ebase[i] + graphs[i].m;
+    if (i > 0 && graphs[i].m &&
+        (graphs[i].u != graphs[i - 1].u + graphs[i - 1].m ||
+         graphs[i].v != graphs[i - 1].v + graphs[i - 1].m ||
+         graphs[i].sign != graphs[i - 1].sign + graphs[i - 1].m))
+      contiguous = false;
   }
+  const uint64_t m = ebase[k];
+  const uint32_t n = vbase[k];
+  if (m >= (1ull << 32)) return fail(c, STC_E_TOO_LARGE, "batch has >= 2^32 lines");
+  c->prepared = false;
+  hipError_t e = hipSuccess;
+  const uint64_t mm = m ? m : 1;
+  if ((e = c->bb.in_u.ensure(mm * 4)) != hipSuccess || (e = c->bb.in_v.ensure(mm * 4)) != hipSuccess ||
+      (e = c->bb.in_s.ensure(mm)) != hipSuccess ||
+      (e = c->batch_vbase.ensure((k + 1) * 4 + (k + 1) * 8 + 64)) != hipSuccess ||
+      (e = c->batch_counts.ensure((uint64_t)k * 16 + 64)) != hipSuccess)
+    return hip_fail(c, e, "batch alloc");
+  uint32_t *du = c->bb.in_u.as<uint32_t>(), *dv = c->bb.in_v.as<uint32_t>();
+  int8_t *ds = c->bb.in_s.as<int8_t>();
+  uint64_t *d_ebase = c->batch_vbase.as<uint64_t>();
+  uint32_t *d_vbase = (uint32_t *)(d_ebase + (k + 1));
+  uint32_t *d_bad = d_vbase + (k + 1);
+  float ms_up = 0.f;
+  e = hipEventRecord(c->ev[0], c->st);
+  // find the first graph with lines for the contiguous fast path
+  uint32_t first = 0;
+  while (first < k && graphs[first].m == 0) first++;
+  if (e == hipSuccess && m && contiguous && first < k) {
+    e = hipMemcpyAsync(du, graphs[first].u, m * 4, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dv, graphs[first].v, m * 4, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, graphs[first].sign, m, hipMemcpyHostToDevice, c->st);
+  } else {
+    for (uint32_t i = 0; i < k && e == hipSuccess; i++) {
+      const uint64_t mi = graphs[i].m, o = ebase[i];
+      if (!mi) continue;
+      e = hipMemcpyAsync(du + o, graphs[i].u, mi * 4, hipMemcpyHostToDevice, c->st);
+      if (e == hipSuccess) e = hipMemcpyAsync(dv + o, graphs[i].v, mi * 4, hipMemcpyHostToDevice, c->st);
+      if (e == hipSuccess) e = hipMemcpyAsync(ds + o, graphs[i].sign, mi, hipMemcpyHostToDevice, c->st);
+    }
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_ebase, ebase.data(), (k + 1) * 8, hipMemcpyHostToDevice, c->st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_vbase, vbase.data(), (k + 1) * 4, hipMemcpyHostToDevice, c->st);
+  if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms_up, c->ev[0], c->ev[1]);
+  if (e != hipSuccess) return hip_fail(c, e, "batch upload");
+  // shift ids, build the union CSR
+  e = hipEventRecord(c->ev[0], c->st);
+  if (e == hipSuccess) e = shift_batch_ids(du, dv, m, d_ebase, d_vbase, k, d_bad, c->st);
+  uint32_t bad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "batch shift");
+  if (bad) return fail(c, STC_E_ARG, "a batch graph has an endpoint >= its n");
+  std::string why;
+  e = build_csr(du, dv, ds, m, n, d_vbase, k, c->bb, c->br, why, c->st);
+  if (e == hipErrorInvalidValue && !why.empty()) return fail(c, STC_E_ARG, why);
+  if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+  float ms_build = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms_build, c->ev[0], c->ev[1]);
+  if (e != hipSuccess) return hip_fail(c, e, "batch csr build");
+  c->g.n = c->br.n;
+  c->g.m = c->br.m;
+  c->g.off = c->bb.off.as<uint32_t>();
+  c->g.col = c->bb.col.as<uint32_t>();
+  c->g.src = c->bb.src.as<uint32_t>();
+  c->prepared = true;
+  std::memset(&c->stats, 0, sizeof(c->stats));
+  c->stats.n = c->br.n;
+  c->stats.m_in = c->br.m_in;
+  c->stats.m_oriented = c->br.m;
+  c->stats.max_out_degree = c->br.max_out;
+  c->stats.b_alg = c->br.b_alg;
+  c->stats.ms_upload = ms_up;
+  c->stats.ms_build = ms_build;
+  // counting phase: eager run, then capture and replay
+  c->hp.mode = dispatch;
+  c->hp.graph_feat[0] = (float)std::log2((double)c->g.n + 1.0);
+  c->hp.graph_feat[1] = (float)std::log2((double)c->g.m + 1.0);
+  e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+  unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
+  const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
+  if (e == hipSuccess)
+    e = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st);
+  std::vector<unsigned long long> eager(2ull * k), got(2ull * k);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(eager.data(), d_pg, 16ull * k, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "batch count");
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  e = hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed);
+  if (e == hipSuccess) {
+    hipError_t e2 = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st);
+    e = hipStreamEndCapture(c->st, &graph);
+    if (e2 != hipSuccess) e = e2;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  std::vector<float> times;
+  for (int r = 0; r < repeat && e == hipSuccess; r++) {
+    e = hipEventRecord(c->ev[0], c->st);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, c->st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    times.push_back(ms);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(got.data(), d_pg, 16ull * k, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess && got != eager) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(c, STC_E_HIP, "graph replay differs from the eager batch count");
+    }
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(c, e, "batch graph");
+  for (uint32_t i = 0; i < k; i++) {
+    out[i].balanced = got[2 * i];
+    out[i].unbalanced = got[2 * i + 1];
+  }
+  c->stats.ms_count = median(times);
+  c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->stats.dispatch_used = (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
+  c->have_classes = false;
+  c->prepared = false;  // the union CSR is not a single graph to recount
   return STC_OK;
 }
 

@@ -444,6 +444,18 @@ struct Acc {
 #pragma unroll
     for (int k = 0; k < 4; k++) wave_add(&ctrl->t[k], c[k]);
   }
+  // Batched counting: add this wave's totals to graph gid's (balanced,
+  // unbalanced) pair and restart from zero.
+  __device__ __forceinline__ void flush_graph(unsigned long long *per_graph, uint32_t gid) {
+    if (CLS) {
+      wave_add(&per_graph[2 * gid], c[3] + c[1]);
+      wave_add(&per_graph[2 * gid + 1], c[2] + c[0]);
+    } else {
+      wave_add(&per_graph[2 * gid], c[0]);
+      wave_add(&per_graph[2 * gid + 1], c[1]);
+    }
+    c[0] = c[1] = c[2] = c[3] = 0;
+  }
 };
 
 // Per-lane probe result folded into wave-uniform edge totals.  q: bit 0 =
@@ -465,51 +477,57 @@ __device__ __forceinline__ void tally(uint32_t q, uint32_t nb, uint32_t &e0, uin
 }
 
 // ---- merge: one lane per edge ----------------------------------------------
+// Two-pointer merge of the part of N+(u) after v with N+(v); per-lane class
+// counters (two-class mode: c[0] balanced, c[1] unbalanced).
+template <bool CLS>
+__device__ __forceinline__ void merge_lane_edge(const DevCsr &g, uint32_t e, uint32_t c[4]) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t w = col[e];
+  const uint32_t a = g.src[e], b = w >> 1, sab = w & 1u;
+  uint32_t pa = e + 1;
+  const uint32_t ea = g.off[a + 1];
+  uint32_t pb = g.off[b];
+  const uint32_t eb = g.off[b + 1];
+  uint32_t x = col[pa], y = col[pb];
+  while (true) {
+    const uint32_t xr = x >> 1, yr = y >> 1;
+    if (xr == yr) {
+      const uint32_t neg = sab + (x & 1u) + (y & 1u);
+      if (CLS) {
+        c[0] += neg == 3;
+        c[1] += neg == 2;
+        c[2] += neg == 1;
+        c[3] += neg == 0;
+      } else {
+        c[0] += (neg & 1u) ^ 1u;
+        c[1] += neg & 1u;
+      }
+      if (++pa >= ea || ++pb >= eb) break;
+      x = col[pa];
+      y = col[pb];
+    } else if (xr < yr) {
+      if (++pa >= ea) break;
+      x = col[pa];
+    } else {
+      if (++pb >= eb) break;
+      y = col[pb];
+    }
+  }
+}
+
 template <bool CLS>
 __global__ __launch_bounds__(256) void k_merge_lane(DevCsr g, Ctrl *ctrl,
                                                     const uint32_t *__restrict__ list) {
   const uint32_t len = ctrl->len[0];
-  // per-lane class counters; in two-class mode c0 = balanced, c1 = unbalanced
-  uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-  const uint32_t *__restrict__ col = g.col;
+  uint32_t c[4] = {0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
-    uint32_t e = list[i];
-    uint32_t w = col[e];
-    uint32_t a = g.src[e], b = w >> 1, sab = w & 1u;
-    uint32_t pa = e + 1, ea = g.off[a + 1];
-    uint32_t pb = g.off[b], eb = g.off[b + 1];
-    uint32_t x = col[pa], y = col[pb];
-    while (true) {
-      uint32_t xr = x >> 1, yr = y >> 1;
-      if (xr == yr) {
-        uint32_t neg = sab + (x & 1u) + (y & 1u);
-        if (CLS) {
-          c0 += neg == 3;
-          c1 += neg == 2;
-          c2 += neg == 1;
-          c3 += neg == 0;
-        } else {
-          c0 += (neg & 1u) ^ 1u;
-          c1 += neg & 1u;
-        }
-        if (++pa >= ea || ++pb >= eb) break;
-        x = col[pa];
-        y = col[pb];
-      } else if (xr < yr) {
-        if (++pa >= ea) break;
-        x = col[pa];
-      } else {
-        if (++pb >= eb) break;
-        y = col[pb];
-      }
-    }
-  }
-  uint64_t v[4] = {c0, c1, c2, c3};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride)
+    merge_lane_edge<CLS>(g, list[i], c);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    for (int d = 32; d > 0; d >>= 1) v[k] += __shfl_down(v[k], d, 64);
-    wave_add(&ctrl->t[k], v[k]);
+    uint64_t v = c[k];
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    wave_add(&ctrl->t[k], v);
   }
 }
 
@@ -518,62 +536,68 @@ __global__ __launch_bounds__(256) void k_merge_lane(DevCsr g, Ctrl *ctrl,
 // current A tile and one of the current B tile; a 6-step ds_bpermute binary
 // search finds each A element in the B tile; the tile whose last element is
 // smaller advances (both on a tie).  The tile after each current tile is
-// loaded one step ahead so its latency overlaps the search.
+// loaded one step ahead so its latency overlaps the search.  e is
+// wave-uniform.
+template <bool CLS>
+__device__ __forceinline__ void merge_wave_edge(const DevCsr &g, uint32_t e, Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  const uint32_t w = col[e];
+  const uint32_t a = g.src[e], b = w >> 1, sab = w & 1u;
+  uint32_t ia = e + 1;
+  const uint32_t ea = g.off[a + 1];
+  uint32_t ib = g.off[b];
+  const uint32_t eb = g.off[b + 1];
+  uint32_t nA = min(64u, ea - ia), nB = min(64u, eb - ib);
+  uint32_t xa = lane < nA ? col[ia + lane] : kEmpty;
+  uint32_t yb = lane < nB ? col[ib + lane] : kEmpty;
+  uint32_t ia2 = ia + nA, ib2 = ib + nB;
+  uint32_t xa2 = ia2 + lane < ea ? col[ia2 + lane] : kEmpty;
+  uint32_t yb2 = ib2 + lane < eb ? col[ib2 + lane] : kEmpty;
+  while (true) {
+    // lower bound of xa among the 64 B entries (padding sorts last)
+    const uint32_t key = xa >> 1;
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t step = 32; step > 0; step >>= 1) {
+      uint32_t cand = (uint32_t)__shfl((int)yb, (int)(pos + step - 1), 64);
+      if ((cand >> 1) < key) pos += step;
+    }
+    const uint32_t hit = (uint32_t)__shfl((int)yb, (int)min(pos, 63u), 64);
+    const bool found = lane < nA && pos < nB && (hit >> 1) == key;
+    acc.add(found, sab + (xa & 1u) + (hit & 1u));
+    // tile ends are wave-uniform: keep them (and the advance) in SGPRs so
+    // the loop runs as uniform control flow with one exit test
+    const uint32_t lastA =
+        __builtin_amdgcn_readfirstlane((uint32_t)__shfl((int)xa, (int)(nA - 1), 64) >> 1);
+    const uint32_t lastB =
+        __builtin_amdgcn_readfirstlane((uint32_t)__shfl((int)yb, (int)(nB - 1), 64) >> 1);
+    if (lastA <= lastB) {
+      ia = ia2;
+      nA = min(64u, ea - ia);
+      xa = xa2;
+      ia2 = ia + nA;
+      xa2 = ia2 + lane < ea ? col[ia2 + lane] : kEmpty;
+    }
+    if (lastB <= lastA) {
+      ib = ib2;
+      nB = min(64u, eb - ib);
+      yb = yb2;
+      ib2 = ib + nB;
+      yb2 = ib2 + lane < eb ? col[ib2 + lane] : kEmpty;
+    }
+    if (ia >= ea || ib >= eb) break;
+  }
+}
+
 template <bool CLS>
 __global__ __launch_bounds__(256) void k_merge_wave(DevCsr g, Ctrl *ctrl,
                                                     const uint32_t *__restrict__ list) {
   const uint32_t len = ctrl->len[1];
-  const uint32_t *__restrict__ col = g.col;
-  const uint32_t lane = __lane_id();
   const uint32_t nw = gridDim.x * (blockDim.x >> 6);
   Acc<CLS> acc;
-  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < len; i += nw) {
-    const uint32_t e = list[i];
-    const uint32_t w = col[e];
-    const uint32_t a = g.src[e], b = w >> 1, sab = w & 1u;
-    uint32_t ia = e + 1;
-    const uint32_t ea = g.off[a + 1];
-    uint32_t ib = g.off[b];
-    const uint32_t eb = g.off[b + 1];
-    uint32_t nA = min(64u, ea - ia), nB = min(64u, eb - ib);
-    uint32_t xa = lane < nA ? col[ia + lane] : kEmpty;
-    uint32_t yb = lane < nB ? col[ib + lane] : kEmpty;
-    uint32_t ia2 = ia + nA, ib2 = ib + nB;
-    uint32_t xa2 = ia2 + lane < ea ? col[ia2 + lane] : kEmpty;
-    uint32_t yb2 = ib2 + lane < eb ? col[ib2 + lane] : kEmpty;
-    while (true) {
-      // lower bound of xa among the 64 B entries (padding sorts last)
-      const uint32_t key = xa >> 1;
-      uint32_t pos = 0;
-#pragma unroll
-      for (uint32_t step = 32; step > 0; step >>= 1) {
-        uint32_t cand = (uint32_t)__shfl((int)yb, (int)(pos + step - 1), 64);
-        if ((cand >> 1) < key) pos += step;
-      }
-      const uint32_t hit = (uint32_t)__shfl((int)yb, (int)min(pos, 63u), 64);
-      const bool found = lane < nA && pos < nB && (hit >> 1) == key;
-      acc.add(found, sab + (xa & 1u) + (hit & 1u));
-      const uint32_t lastA = (uint32_t)__shfl((int)xa, (int)(nA - 1), 64) >> 1;
-      const uint32_t lastB = (uint32_t)__shfl((int)yb, (int)(nB - 1), 64) >> 1;
-      const bool needA = lastA <= lastB, needB = lastB <= lastA;
-      if (needA) {
-        ia = ia2;
-        if (ia >= ea) break;
-        nA = min(64u, ea - ia);
-        xa = xa2;
-        ia2 = ia + nA;
-        xa2 = ia2 + lane < ea ? col[ia2 + lane] : kEmpty;
-      }
-      if (needB) {
-        ib = ib2;
-        if (ib >= eb) break;
-        nB = min(64u, eb - ib);
-        yb = yb2;
-        ib2 = ib + nB;
-        yb2 = ib2 + lane < eb ? col[ib2 + lane] : kEmpty;
-      }
-    }
-  }
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < len; i += nw)
+    merge_wave_edge<CLS>(g, __builtin_amdgcn_readfirstlane(list[i]), acc);
   acc.flush(ctrl);
 }
 
@@ -692,58 +716,132 @@ __device__ __forceinline__ int table_log2(uint32_t d, int min_lg) {
   return lg < min_lg ? min_lg : lg;
 }
 
+// Stage N+(a) in an LDS hash table and count a's hash edges; called by every
+// thread of the workgroup (T holds at least 4 * d slots).
+template <bool CLS>
+__device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, uint32_t *T,
+                                                  const uint8_t *table, const TuneParams &tune,
+                                                  Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthreads >> 6;
+  const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
+  const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  const uint32_t span = span_of(col, la, ea);
+  const int lg = table_log2(d, 6);
+  const uint32_t cap = 1u << lg, mask = cap - 1u;
+  for (uint32_t k = tid; k < cap; k += nthreads) T[k] = kEmpty;
+  __syncthreads();
+  for (uint32_t k = tid; k < d; k += nthreads) {
+    uint32_t x = col[la + k];
+    uint32_t h = (x >> 1) & mask;
+    while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
+  }
+  __syncthreads();
+  walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
+    if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
+    stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
+  });
+  __syncthreads();
+}
+
+// One wave stages N+(a) (at most kWaveMaxDeg entries) in its own table T of
+// kWaveCap slots and counts a's hash edges.
+constexpr uint32_t kWaveCap = 4 * kWaveMaxDeg;
+
+template <bool CLS>
+__device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, uint32_t *T,
+                                                 const uint8_t *table, const TuneParams &tune,
+                                                 Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
+  const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  const uint32_t span = span_of(col, la, ea);
+  const int lg = table_log2(d, 4);
+  const uint32_t cap = 1u << lg, mask = cap - 1u;
+  for (uint32_t k = lane; k < cap; k += 64) T[k] = kEmpty;
+  wave_lds_sync();
+  if (lane < d) {
+    uint32_t x = col[la + lane];
+    uint32_t h = (x >> 1) & mask;
+    while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
+  }
+  wave_lds_sync();
+  walk_edges(g, la, ea, 1u, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
+    if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
+    stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
+  });
+  wave_lds_sync();
+}
+
+// Stage N+(a) as a 2-bit (present, negative) LDS bitmap over its rank span:
+// entry r (bits 2r, 2r+1) holds rank lo + r, so a probe is one LDS read.  B
+// must be zero on entry and is zero again on return (only the words the
+// vertex touched are cleared).  Called by every thread of the workgroup.
+template <bool CLS>
+__device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a, uint32_t *B,
+                                                    const uint8_t *table,
+                                                    const TuneParams &tune, Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthreads >> 6;
+  const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
+  const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
+  const uint32_t span = span_of(col, la, ea);
+  for (uint32_t k = tid; k < d; k += nthreads) {
+    const uint32_t x = col[la + k];
+    const uint32_t r = (x >> 1) - lo;
+    atomicOr(&B[r >> 4], (1u | ((x & 1u) << 1)) << ((r & 15u) * 2u));
+  }
+  __syncthreads();
+  walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
+    if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_BITMAP) return;
+    stream_bitmap<CLS>(col, pb, eb, w & 1u, B, lo << 1, span << 1, acc);
+  });
+  __syncthreads();
+  for (uint32_t k = tid; k < d; k += nthreads) B[((col[la + k] >> 1) - lo) >> 4] = 0u;
+  __syncthreads();
+}
+
+// Pop one item for the whole workgroup from an XCD queue.
+__device__ __forceinline__ bool block_pop(uint32_t (*ctr)[32], uint32_t len, QueueCursor &cur,
+                                          uint32_t *s_first, uint32_t *s_count,
+                                          uint32_t &first) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t f = 0, c = 0;
+    if (!queue_pop(ctr, len, 1u, cur, f, c)) c = 0;
+    *s_first = f;
+    *s_count = c;
+  }
+  __syncthreads();
+  first = __builtin_amdgcn_readfirstlane(*s_first);
+  return *s_count != 0;
+}
+
 // One workgroup per source vertex (source lists longer than one wave).
 template <bool CLS>
 __global__ __launch_bounds__(kVertexBlock) void k_hash(DevCsr g, Ctrl *ctrl,
-                                              const DispatchParams *P,
-                                              const uint32_t *__restrict__ vlist) {
+                                                       const DispatchParams *P,
+                                                       const uint32_t *__restrict__ vlist) {
   extern __shared__ uint32_t T[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
-  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nwaves = blockDim.x >> 6;
-  if (tid < kBuckets) table[tid] = ctrl->table[tid];
-  if (tid == 0) tune = P->tune;
+  if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
+  if (threadIdx.x == 0) tune = P->tune;
   const uint32_t len = ctrl->len[2];
-  const uint32_t *__restrict__ col = g.col;
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
-  while (true) {
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t f = 0, c = 0;
-      if (!queue_pop(ctrl->queue[0], len, 1u, cur, f, c)) c = 0;
-      s_first = f;
-      s_count = c;
-    }
-    __syncthreads();
-    if (s_count == 0) break;
-    const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[__builtin_amdgcn_readfirstlane(s_first)]);
-    const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
-    const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
-    const uint32_t span = span_of(col, la, ea);
-    const int lg = table_log2(d, 6);
-    const uint32_t cap = 1u << lg, mask = cap - 1u;
-    for (uint32_t k = tid; k < cap; k += blockDim.x) T[k] = kEmpty;
-    __syncthreads();
-    for (uint32_t k = tid; k < d; k += blockDim.x) {
-      uint32_t x = col[la + k];
-      uint32_t h = (x >> 1) & mask;
-      while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
-    }
-    __syncthreads();
-    walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
-      if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
-      stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
-    });
-  }
+  uint32_t first;
+  while (block_pop(ctrl->queue[0], len, cur, &s_first, &s_count, first))
+    hash_vertex_block<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first]), T, table, tune, acc);
   acc.flush(ctrl);
 }
 
-// One wave per source vertex (source lists of at most kWaveMaxDeg entries):
-// each wave owns a kWaveCap-slot table.
-constexpr uint32_t kWaveCap = 4 * kWaveMaxDeg;
+// One wave per source vertex (source lists of at most kWaveMaxDeg entries).
 constexpr uint32_t kWaveChunk = 8;  // vertices per queue pop
 
 template <bool CLS>
@@ -753,14 +851,13 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   __shared__ uint32_t Tall[4 * kWaveCap];
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int wave = threadIdx.x >> 6;
   const uint32_t lane = __lane_id();
-  table[tid] = ctrl->table[tid];
-  if (tid == 0) tune = P->tune;
+  table[threadIdx.x] = ctrl->table[threadIdx.x];
+  if (threadIdx.x == 0) tune = P->tune;
   __syncthreads();
   uint32_t *T = Tall + wave * kWaveCap;
   const uint32_t len = ctrl->len[4];
-  const uint32_t *__restrict__ col = g.col;
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   while (true) {
@@ -769,34 +866,14 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
     first = __shfl(first, 0, 64);
     count = __shfl(count, 0, 64);
     if (count == 0) break;
-    for (uint32_t it = 0; it < count; it++) {
-      const uint32_t a = vlist[first + it];
-      const uint32_t la = g.off[a], ea = g.off[a + 1], d = ea - la;
-      const uint32_t span = span_of(col, la, ea);
-      const int lg = table_log2(d, 4);
-      const uint32_t cap = 1u << lg, mask = cap - 1u;
-      for (uint32_t k = lane; k < cap; k += 64) T[k] = kEmpty;
-      wave_lds_sync();
-      if (lane < d) {
-        uint32_t x = col[la + lane];
-        uint32_t h = (x >> 1) & mask;
-        while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
-      }
-      wave_lds_sync();
-      walk_edges(g, la, ea, 1u, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
-        if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
-        stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
-      });
-      wave_lds_sync();
-    }
+    for (uint32_t it = 0; it < count; it++)
+      hash_vertex_wave<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first + it]), T, table, tune,
+                            acc);
   }
   acc.flush(ctrl);
 }
 
-// ---- vertex-grouped: LDS 2-bit bitmap over N+(u)'s rank span ---------------
-// Entry r (bits 2r, 2r+1 of the bitmap) holds (present, negative) for rank
-// lo + r, so a probe is one LDS read.  Words touched by a vertex are cleared
-// after it, so the bitmap is zeroed only once per workgroup.
+// One workgroup per source vertex: LDS bitmap (zeroed once per workgroup).
 template <bool CLS>
 __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                          const DispatchParams *P,
@@ -806,44 +883,112 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
-  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int nwaves = blockDim.x >> 6;
-  if (tid < kBuckets) table[tid] = ctrl->table[tid];
-  if (tid == 0) tune = P->tune;
-  for (uint32_t k = tid; k < bitmap_words; k += blockDim.x) B[k] = 0u;
+  if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
+  if (threadIdx.x == 0) tune = P->tune;
+  for (uint32_t k = threadIdx.x; k < bitmap_words; k += blockDim.x) B[k] = 0u;
   const uint32_t len = ctrl->len[3];
-  const uint32_t *__restrict__ col = g.col;
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
-  while (true) {
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t f = 0, c = 0;
-      if (!queue_pop(ctrl->queue[1], len, 1u, cur, f, c)) c = 0;
-      s_first = f;
-      s_count = c;
-    }
-    __syncthreads();
-    if (s_count == 0) break;
-    const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[__builtin_amdgcn_readfirstlane(s_first)]);
-    const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
-    const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
-    const uint32_t span = span_of(col, la, ea);
-    for (uint32_t k = tid; k < d; k += blockDim.x) {
-      const uint32_t x = col[la + k];
-      const uint32_t r = (x >> 1) - lo;
-      atomicOr(&B[r >> 4], (1u | ((x & 1u) << 1)) << ((r & 15u) * 2u));
-    }
-    __syncthreads();
-    walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
-      if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_BITMAP) return;
-      stream_bitmap<CLS>(col, pb, eb, w & 1u, B, lo << 1, span << 1, acc);
-    });
-    __syncthreads();
-    for (uint32_t k = tid; k < d; k += blockDim.x) B[((col[la + k] >> 1) - lo) >> 4] = 0u;
-  }
+  uint32_t first;
+  while (block_pop(ctrl->queue[1], len, cur, &s_first, &s_count, first))
+    bitmap_vertex_block<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first]), B, table, tune,
+                             acc);
   acc.flush(ctrl);
+}
+
+// ---- batched counting: one persistent kernel over the union of k graphs ----
+// Items, in queue order (popped from the top, so heaviest kinds first):
+//   merge-lane chunks of kLaneItem edges (one per thread), merge-wave chunks
+//   of one edge per wave, wave-hash chunks of one vertex per wave, hash
+//   vertices, bitmap vertices.  Every item's triangles are added to the
+//   counters of the graph owning its source vertex (graph_of is indexed by
+//   rank; each graph's ranks are one contiguous range).
+constexpr uint32_t kBatchBlock = 256;
+constexpr uint32_t kLaneItem = kBatchBlock;
+constexpr uint32_t kWavesPerBatchBlock = kBatchBlock / 64;
+
+__device__ __forceinline__ void lane_flush_graph(unsigned long long *per_graph, bool valid,
+                                                 uint32_t gid, uint32_t c0, uint32_t c1) {
+  const uint64_t vm = __ballot(valid);
+  if (!vm) return;
+  const uint32_t g0 = __shfl(gid, __builtin_ctzll(vm), 64);
+  if (__ballot(valid && gid == g0) == vm) {
+    uint64_t b = valid ? c0 : 0u, u = valid ? c1 : 0u;
+    for (int d = 32; d > 0; d >>= 1) {
+      b += __shfl_down(b, d, 64);
+      u += __shfl_down(u, d, 64);
+    }
+    wave_add(&per_graph[2 * g0], b);
+    wave_add(&per_graph[2 * g0 + 1], u);
+  } else if (valid) {
+    if (c0) atomicAdd(&per_graph[2 * gid], (unsigned long long)c0);
+    if (c1) atomicAdd(&per_graph[2 * gid + 1], (unsigned long long)c1);
+  }
+}
+
+__global__ __launch_bounds__(kBatchBlock) void k_batch(
+    DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
+    const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vh,
+    const uint32_t *__restrict__ vb, const uint32_t *__restrict__ vw,
+    const uint32_t *__restrict__ graph_of, unsigned long long *__restrict__ per_graph,
+    uint32_t bitmap_words, uint32_t hash_slots) {
+  extern __shared__ uint32_t lds[];
+  uint32_t *B = lds;                                  // bitmap, kept zero between items
+  uint32_t *T = lds + bitmap_words;                   // hash table (hash_slots)
+  uint32_t *TW = T + hash_slots;                      // per-wave tables
+  __shared__ uint8_t table[kBuckets];
+  __shared__ uint32_t s_first, s_count;
+  __shared__ TuneParams tune;
+  const uint32_t tid = threadIdx.x, lane = __lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  table[tid] = ctrl->table[tid];
+  if (tid == 0) tune = P->tune;
+  for (uint32_t k = tid; k < bitmap_words; k += blockDim.x) B[k] = 0u;
+  const uint32_t nl = ctrl->len[0], nwv = ctrl->len[1], nh = ctrl->len[2], nb = ctrl->len[3],
+                 nhw = ctrl->len[4];
+  const uint32_t i_lane = (nl + kLaneItem - 1) / kLaneItem;
+  const uint32_t i_wave = (nwv + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
+  const uint32_t i_hw = (nhw + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
+  const uint32_t b1 = i_lane, b2 = b1 + i_wave, b3 = b2 + i_hw, b4 = b3 + nh, total = b4 + nb;
+  QueueCursor cur{blockIdx.x & 7u, 0u};
+  Acc<false> acc;
+  uint32_t it;
+  while (block_pop(ctrl->queue[3], total, cur, &s_first, &s_count, it)) {
+    if (it < b1) {  // merge-lane chunk
+      const uint32_t i = it * kLaneItem + tid;
+      const bool valid = i < nl;
+      uint32_t c[4] = {0, 0, 0, 0};
+      uint32_t gid = 0;
+      if (valid) {
+        const uint32_t e = lane_list[i];
+        gid = graph_of[g.src[e]];
+        merge_lane_edge<false>(g, e, c);
+      }
+      lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
+    } else if (it < b2) {  // merge-wave chunk
+      const uint32_t i = (it - b1) * kWavesPerBatchBlock + wave;
+      if (i < nwv) {
+        const uint32_t e = __builtin_amdgcn_readfirstlane(wave_list[i]);
+        merge_wave_edge<false>(g, e, acc);
+        acc.flush_graph(per_graph, graph_of[g.src[e]]);
+      }
+    } else if (it < b3) {  // wave-hash chunk
+      const uint32_t i = (it - b2) * kWavesPerBatchBlock + wave;
+      if (i < nhw) {
+        const uint32_t a = __builtin_amdgcn_readfirstlane(vw[i]);
+        hash_vertex_wave<false>(g, a, TW + wave * kWaveCap, table, tune, acc);
+        acc.flush_graph(per_graph, graph_of[a]);
+      }
+    } else if (it < b4) {  // hash vertex
+      const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b3]);
+      hash_vertex_block<false>(g, a, T, table, tune, acc);
+      acc.flush_graph(per_graph, graph_of[a]);
+    } else {  // bitmap vertex
+      const uint32_t a = __builtin_amdgcn_readfirstlane(vb[it - b4]);
+      bitmap_vertex_block<false>(g, a, B, table, tune, acc);
+      acc.flush_graph(per_graph, graph_of[a]);
+    }
+  }
 }
 
 __global__ void k_fill_words(uint32_t *p, uint64_t n, uint32_t v) {
@@ -874,13 +1019,33 @@ size_t bitmap_lds_bytes(const TuneParams &t) {
     if (e_ != hipSuccess) return e_; \
   } while (0)
 
-// Buffers: cb.gcount holds Ctrl; cb.lists the merge-lane and merge-wave edge
-// lists (m words each); cb.vflag n words kept zero between runs; cb.vlist 6n
-// words (hash flags, their scan, bitmap flags, their scan, the two vertex
-// lists).
-hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
-                         const DispatchParams *dp, CountBuffers &cb, bool classes,
-                         int num_cus, hipStream_t st, const hipEvent_t *marks) {
+// Raise the dynamic-LDS limit of the kernels that use more than 64 KB; called
+// once per process before any launch (never inside a stream capture).
+hipError_t init_kernel_attributes() {
+  static bool done = false;
+  if (done) return hipSuccess;
+  const void *fns[] = {(const void *)k_bitmap<true>, (const void *)k_bitmap<false>,
+                       (const void *)k_hash<true>, (const void *)k_hash<false>,
+                       (const void *)k_batch};
+  for (const void *f : fns)
+    STC_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+  done = true;
+  return hipSuccess;
+}
+
+namespace {
+
+struct BinOut {
+  uint32_t *lane_list, *wave_list, *vh, *vb, *vw;
+};
+
+// Buffers: cb.gcount holds Ctrl; cb.lists the per-workgroup segments and the
+// packed merge-lane / merge-wave edge lists; cb.vflag n words kept zero
+// between runs; cb.vlist 9n words (flags, their scans and the three vertex
+// lists).  Every buffer is sized before the first launch, so a second call
+// with the same graph allocates nothing (and can be captured in a graph).
+hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
+                           int num_cus, hipStream_t st, const hipEvent_t *marks, BinOut &out) {
   const uint64_t m = g.m;
   const uint32_t n = g.n;
   STC_TRY(cb.gcount.ensure(sizeof(Ctrl)));
@@ -901,15 +1066,17 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
   uint32_t *lane_seg = cb.lists.as<uint32_t>();
   uint32_t *wave_seg = lane_seg + seg_words;
-  uint32_t *lane_list = wave_seg + seg_words;
-  uint32_t *wave_list = lane_list + (m ? m : 1);
+  out.lane_list = wave_seg + seg_words;
+  out.wave_list = out.lane_list + (m ? m : 1);
   uint32_t *blk_cnt = cb.blk.as<uint32_t>();
   uint32_t *blk_pos = blk_cnt + 2 * bin_grid;
   uint32_t *vf = cb.vflag.as<uint32_t>();
   uint32_t *fh = cb.vlist.as<uint32_t>();
   uint32_t *ph = fh + (n + 1), *fb = ph + (n + 1), *pb = fb + (n + 1);
-  uint32_t *vh = pb + (n + 1), *vb = vh + (n + 1);
-  uint32_t *fw = vb + (n + 1), *pw = fw + (n + 1), *vw = pw + (n + 1);
+  out.vh = pb + (n + 1);
+  out.vb = out.vh + (n + 1);
+  uint32_t *fw = out.vb + (n + 1), *pw = fw + (n + 1);
+  out.vw = pw + (n + 1);
 
   if (marks) STC_TRY(hipEventRecord(marks[0], st));
   hipLaunchKernelGGL(k_fill_words, dim3(8), dim3(256), 0, st, (uint32_t *)ctrl,
@@ -922,7 +1089,8 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(blk_cnt, blk_pos, 2 * (uint64_t)bin_grid, cb.scan, st));
   hipLaunchKernelGGL(k_list_compact, dim3(2 * bin_grid), dim3(256), 0, st, lane_seg, wave_seg,
-                     seg, blk_cnt, blk_pos, (uint32_t)bin_grid, lane_list, wave_list, ctrl);
+                     seg, blk_cnt, blk_pos, (uint32_t)bin_grid, out.lane_list, out.wave_list,
+                     ctrl);
   STC_TRY(hipGetLastError());
   if (n > 0) {
     const int vgrid = grid_for(n, 256, num_cus * 8);
@@ -933,58 +1101,80 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     STC_TRY(scan_exclusive_u32(fb, pb, n, cb.scan, st));
     STC_TRY(scan_exclusive_u32(fw, pw, n, cb.scan, st));
     hipLaunchKernelGGL(k_vlist_scatter, dim3(vgrid), dim3(256), 0, st, fh, ph, fb, pb, fw,
-                       pw, n, vh, vb, vw, ctrl);
+                       pw, n, out.vh, out.vb, out.vw, ctrl);
     STC_TRY(hipGetLastError());
   }
   if (marks) STC_TRY(hipEventRecord(marks[1], st));
+  return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
+                         const DispatchParams *dp, CountBuffers &cb, bool classes,
+                         int num_cus, hipStream_t st, const hipEvent_t *marks) {
+  BinOut L;
+  STC_TRY(enqueue_binning(g, dp, cb, num_cus, st, marks, L));
+  Ctrl *ctrl = cb.gcount.as<Ctrl>();
   const int grid = num_cus * 8;
   const size_t hash_lds = hash_lds_bytes(hp.tune);
   const size_t bm_lds = bitmap_lds_bytes(hp.tune);
   const int hash_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (128 * 1024) / hash_lds));
   const int bm_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (144 * 1024) / bm_lds));
   const uint32_t bm_words = (1u << hp.tune.bitmap_span_log2) / 16u;
-  static bool attr_done = false;
-  if (!attr_done) {
-    STC_TRY(hipFuncSetAttribute((const void *)k_bitmap<true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    STC_TRY(hipFuncSetAttribute((const void *)k_bitmap<false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    STC_TRY(hipFuncSetAttribute((const void *)k_hash<true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    STC_TRY(hipFuncSetAttribute((const void *)k_hash<false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-    attr_done = true;
-  }
   auto mark = [&](int i) -> hipError_t {
     return marks ? hipEventRecord(marks[i], st) : hipSuccess;
   };
   if (classes) {
-    hipLaunchKernelGGL(k_merge_lane<true>, dim3(grid), dim3(256), 0, st, g, ctrl, lane_list);
+    hipLaunchKernelGGL(k_merge_lane<true>, dim3(grid), dim3(256), 0, st, g, ctrl, L.lane_list);
     STC_TRY(mark(2));
-    hipLaunchKernelGGL(k_merge_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
+    hipLaunchKernelGGL(k_merge_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, L.wave_list);
     STC_TRY(mark(3));
-    hipLaunchKernelGGL(k_hash_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
+    hipLaunchKernelGGL(k_hash_wave<true>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, L.vw);
     STC_TRY(mark(4));
     hipLaunchKernelGGL(k_hash<true>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
-                       dp, vh);
+                       dp, L.vh);
     STC_TRY(mark(5));
     hipLaunchKernelGGL(k_bitmap<true>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
-                       dp, vb, bm_words);
+                       dp, L.vb, bm_words);
     STC_TRY(mark(6));
   } else {
-    hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, lane_list);
+    hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, L.lane_list);
     STC_TRY(mark(2));
-    hipLaunchKernelGGL(k_merge_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, wave_list);
+    hipLaunchKernelGGL(k_merge_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, L.wave_list);
     STC_TRY(mark(3));
-    hipLaunchKernelGGL(k_hash_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, vw);
+    hipLaunchKernelGGL(k_hash_wave<false>, dim3(grid), dim3(256), 0, st, g, ctrl, dp, L.vw);
     STC_TRY(mark(4));
     hipLaunchKernelGGL(k_hash<false>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
-                       dp, vh);
+                       dp, L.vh);
     STC_TRY(mark(5));
     hipLaunchKernelGGL(k_bitmap<false>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
-                       dp, vb, bm_words);
+                       dp, L.vb, bm_words);
     STC_TRY(mark(6));
   }
+  return hipGetLastError();
+}
+
+size_t batch_lds_bytes(const TuneParams &t) {
+  return 4u * ((size_t)((1u << t.bitmap_span_log2) / 16u) + ((size_t)4u << t.hash_max_log2) +
+               kWavesPerBatchBlock * kWaveCap);
+}
+
+hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
+                         CountBuffers &cb, const uint32_t *graph_of,
+                         unsigned long long *per_graph, uint32_t k, int num_cus,
+                         hipStream_t st) {
+  BinOut L;
+  STC_TRY(enqueue_binning(g, dp, cb, num_cus, st, nullptr, L));
+  hipLaunchKernelGGL(k_fill_words, dim3(grid_for(4ull * k, 256, 4096)), dim3(256), 0, st,
+                     (uint32_t *)per_graph, 4ull * k, 0u);
+  STC_TRY(hipGetLastError());
+  const size_t lds = batch_lds_bytes(hp.tune);
+  const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (150 * 1024) / lds));
+  hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
+                     cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vb, L.vw,
+                     graph_of, per_graph, (1u << hp.tune.bitmap_span_log2) / 16u,
+                     4u << hp.tune.hash_max_log2);
   return hipGetLastError();
 }
 

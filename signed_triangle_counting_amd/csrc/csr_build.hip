@@ -335,6 +335,29 @@ inline int bits_for(uint64_t x) {  // bits needed to represent values < x
   return b;
 }
 
+// Batched input: line i of the concatenated edge list belongs to graph
+// g = upper_bound(ebase, i) - 1; its ids must be < vbase[g+1] - vbase[g] and
+// are shifted by vbase[g] into the union's id space.
+__global__ void k_shift_ids(uint32_t *__restrict__ u, uint32_t *__restrict__ v, uint64_t m,
+                            const unsigned long long *__restrict__ ebase,
+                            const uint32_t *__restrict__ vbase, uint32_t k,
+                            uint32_t *__restrict__ bad) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = k;  // find the last g with ebase[g] <= i
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (ebase[mid] <= i) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t n = vbase[lo + 1] - vbase[lo];
+    const uint32_t a = u[i], b = v[i];
+    if (a >= n || b >= n) atomicOr(bad, 1u);
+    u[i] = a + vbase[lo];
+    v[i] = b + vbase[lo];
+  }
+}
+
 }  // namespace
 
 #define STC_TRY(expr)                         \
@@ -482,7 +505,7 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
   // rank by (graph, degree, id)
   uint32_t *graph_of = nullptr;
   int bg = 0;
-  if (vbase_dev && k > 1) {
+  if (vbase_dev && k >= 1) {
     STC_TRY(bb.graph_of.ensure(((uint64_t)n + 1) * 4));
     graph_of = bb.graph_of.as<uint32_t>();
     hipLaunchKernelGGL(k_graph_of, dim3(k), dim3(T), 0, st, vbase_dev, k, graph_of);
@@ -540,6 +563,15 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
   res.max_out = mo;
   res.b_alg = 4.0 * (double)words + 4.0 * (double)m + 8.0 * ((double)n + 1.0);
   return hipSuccess;
+}
+
+hipError_t shift_batch_ids(uint32_t *u, uint32_t *v, uint64_t m, const uint64_t *ebase_dev,
+                           const uint32_t *vbase_dev, uint32_t k, uint32_t *bad_dev,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, bad_dev, (uint64_t)1, 0u);
+  hipLaunchKernelGGL(k_shift_ids, dim3(grid_for(m, 256)), dim3(256), 0, st, u, v, m,
+                     (const unsigned long long *)ebase_dev, vbase_dev, k, bad_dev);
+  return hipGetLastError();
 }
 
 }  // namespace stc

@@ -150,6 +150,13 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
                      uint32_t k, BuildBuffers &bb, BuildResult &res,
                      std::string &err, hipStream_t st);
 
+// Shift the ids of k concatenated edge lists into the union id space (graph g
+// owns lines [ebase[g], ebase[g+1]) and ids [vbase[g], vbase[g+1])); sets
+// *bad_dev nonzero if a line has an id outside its graph.
+hipError_t shift_batch_ids(uint32_t *u, uint32_t *v, uint64_t m, const uint64_t *ebase_dev,
+                           const uint32_t *vbase_dev, uint32_t k, uint32_t *bad_dev,
+                           hipStream_t st);
+
 struct CountBuffers {
   DevBuf gcount, lists, blk, vflag, vlist;
   ScanScratch scan;
@@ -179,12 +186,15 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r,
 size_t hash_lds_bytes(const TuneParams &t);
 size_t bitmap_lds_bytes(const TuneParams &t);
 
-// Persistent batched counting: one launch over the union CSR.
-hipError_t enqueue_batch_persistent(const DevCsr &g,
-                                    const DispatchParams *params_dev,
-                                    CountBuffers &cb, const uint32_t *vbase,
-                                    uint32_t k, uint64_t *per_graph,
-                                    int num_cus, hipStream_t st);
+// Batched counting over the union CSR of k graphs: binning, then one
+// persistent kernel; per_graph gets k (balanced, unbalanced) pairs.
+// graph_of maps a rank to its graph.
+hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
+                         CountBuffers &cb, const uint32_t *graph_of,
+                         unsigned long long *per_graph, uint32_t k, int num_cus,
+                         hipStream_t st);
+size_t batch_lds_bytes(const TuneParams &t);
+hipError_t init_kernel_attributes();
 
 // MLP training on device (full-batch Adam in one wave64, fixed reduction
 // order -> deterministic).  x: nsamp x kMlpIn normalised features; r: nsamp x

@@ -209,3 +209,79 @@ def test_cli_output(tmp_path, goldens):
     assert "[Triangle Counting Execution] took" in out
     bad = subprocess.run([stc.CLI_PATH], stdout=subprocess.PIPE, text=True, timeout=60)
     assert bad.returncode == 1 and "Usage" in bad.stdout
+
+
+# ---- batched counting (one persistent launch, captured in a hipGraph) -------
+def test_batch_small_vs_oracle(ctx):
+    rng = np.random.default_rng(77)
+    graphs = []
+    for i in range(40):
+        n = int(rng.integers(1, 400))
+        m = int(rng.integers(0, 6 * n + 1))
+        if n < 2:
+            m = 0
+        u, v, s = random_graph(rng, max(n, 1), m, dup=0.2, loops=0.1) if m else (
+            np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.int8))
+        graphs.append(stc.Graph(u=u, v=v, sign=s, n=n))
+    graphs.append(stc.gen_rmat(13, 16, 0.57, 0.19, 0.19, 0.3, 5))
+    graphs.append(stc.Graph(u=[], v=[], sign=[], n=0))
+    want = [oracle_counts(g.n, g.u, g.v, g.sign) for g in graphs]
+    for mode in MODES:
+        say("batch small", mode)
+        got = ctx.count_batch(graphs, dispatch=mode, repeat=3)
+        assert got == want, mode
+
+
+def test_batch_rejects_out_of_range_id(ctx):
+    graphs = [stc.Graph(u=[0, 1], v=[1, 2], sign=[1, 1], n=3),
+              stc.Graph(u=[0, 3], v=[1, 2], sign=[1, 1], n=3)]
+    with pytest.raises(stc.StcError) as ei:
+        ctx.count_batch(graphs)
+    assert ei.value.code == stc.STC_E_ARG
+
+
+def test_batch_workload_vs_reference_goldens(ctx, goldens):
+    """4,096 graphs, per-graph counts against the oracle fixture that the
+    reference programs pin on a sample (tests/golden/make_goldens.py)."""
+    say("batch workload generate")
+    b = stc.gen_workload_batch(4096)
+    per = goldens["batch"]["per_graph"]
+    assert b.total_m == goldens["batch"]["total_m"]
+    for mode in ("learned", "fixed"):
+        say("batch workload", mode)
+        got = ctx.count_batch(b, dispatch=mode, repeat=2)
+        bad = [i for i in range(b.k) if list(got[i]) != per[i][1:3]]
+        assert not bad, (mode, bad[:10])
+
+
+@pytest.mark.parametrize("scale,ef", [(16, 16), (18, 24)])
+def test_partial_tables_strategy_invariant(ctx, scale, ef):
+    """Counting any subset of buckets gives the same partial counts under
+    every strategy (each triangle is counted by exactly one oriented edge),
+    per bucket group of the fixed rule and for single heavy buckets."""
+    g = stc.gen_rmat(scale, ef, 0.57, 0.19, 0.19, 0.3, 300 + scale)
+    ctx.prepare(g)
+    hist = ctx.bucket_histogram()
+    nz = [int(b) for b in np.nonzero(hist)[0]]
+    groups = {}
+    for b in nz:
+        la, lb = b // 16, b % 16
+        key = "hash" if abs(la - lb) >= 2 else ("bitmap" if la >= 9 else "merge")
+        groups.setdefault(key, []).append(b)
+    heavy = sorted(nz, key=lambda b: -int(hist[b]))[:6]
+    sets = list(groups.values()) + [[b] for b in heavy]
+    total = (0, 0)
+    for bs in sets:
+        res = {}
+        for s in ("merge", "hash", "bitmap"):
+            t = np.zeros(256, np.uint8)
+            t[bs] = stc.TABLE[s]
+            res[s] = ctx.count_with_table(t)
+        say("partial", scale, bs[:4], res)
+        assert len(set(res.values())) == 1, (bs, res)
+    for bs in groups.values():
+        t = np.zeros(256, np.uint8)
+        t[bs] = stc.TABLE["merge"]
+        c = ctx.count_with_table(t)
+        total = (total[0] + c[0], total[1] + c[1])
+    assert total == oracle_counts(g.n, g.u, g.v, g.sign)
