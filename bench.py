@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap"]
+KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitmap"]
 WORKLOADS = {1: "er_10k_100k", 2: "grid_960x960_diag", 3: "rmat22_ef16", 4: "hub_18m"}
 
 
@@ -124,7 +124,8 @@ def traffic_from(pmc_dir, workload, out_path):
     fw = copy_factor("copy_write", "WRITE_SIZE")
     fetch, write = read("fetch"), read("write")
     names = {"k_merge_lane": "merge_lane", "k_merge_wave": "merge_wave",
-             "k_hash_wave": "hash_wave", "k_hash<": "hash", "k_bitmap": "bitmap"}
+             "k_hash_wave": "hash_wave", "k_hash<": "hash", "k_bitmap_wave": "bitmap_wave",
+             "k_bitmap<": "bitmap"}
     out = {}
     try:
         with open(out_path) as f:

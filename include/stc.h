@@ -35,7 +35,7 @@ extern "C" {
 #ifndef STC_OK
 #define STC_OK 0
 #define STC_E_ARG (-1)       /* bad argument, or an id >= n */
-#define STC_E_TOO_LARGE (-2) /* n > 2^31, or >= 2^32 input lines/edges */
+#define STC_E_TOO_LARGE (-2) /* n > 2^31 - 2^20, or > 2^32 - 2^20 input lines */
 #define STC_E_OOM (-3)       /* device allocation failed */
 #define STC_E_HIP (-4)       /* HIP runtime error */
 #define STC_E_MODEL (-5)     /* dispatcher model missing/invalid: fixed used */
@@ -101,10 +101,11 @@ typedef struct stc_stats {
   uint64_t edges_bitmap;
   uint64_t edges_skipped;  /* edges that cannot close a triangle */
   int dispatch_used;       /* stc_dispatch actually applied */
-  float ms_kernel[5];      /* merge-lane, merge-wave, wave-hash, workgroup-hash
-                              and bitmap kernel time (hipEvent on the library
-                              stream), median over repeats */
-  double bytes_kernel[5];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
+  float ms_kernel[6];      /* merge-lane, merge-wave, wave-hash, workgroup-hash,
+                              wave-bitmap and workgroup-bitmap kernel time
+                              (hipEvent on the library stream), median over
+                              repeats */
+  double bytes_kernel[6];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
                               oriented edges each of those kernels counted */
 } stc_stats;
 

@@ -32,7 +32,7 @@ STC_E_NODEV = -7
 
 DISPATCH = {"learned": 0, "fixed": 1, "merge": 2, "hash": 3, "bitmap": 4}
 
-KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap"]
+KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitmap"]
 
 # Names every build of libstc.so must export (the declarations of
 # include/stc.h and include/stc_gen.h).
@@ -97,7 +97,7 @@ class Stats(C.Structure):
                 ("ms_bin", C.c_float), ("edges_merge", C.c_uint64),
                 ("edges_hash", C.c_uint64), ("edges_bitmap", C.c_uint64),
                 ("edges_skipped", C.c_uint64), ("dispatch_used", C.c_int),
-                ("ms_kernel", C.c_float * 5), ("bytes_kernel", C.c_double * 5)]
+                ("ms_kernel", C.c_float * 6), ("bytes_kernel", C.c_double * 6)]
 
     def as_dict(self):
         d = {}

@@ -7,8 +7,10 @@ distinct from the benchmark workloads):
   * merge_wave_min_log2  the list length from which one wave64 per edge
                           (tiled merge) replaces one lane per edge, per
                           shorter-side bucket;
-  * bitmap_span_log2     rank-span capacity of the LDS bitmap (LDS size and
-                          so workgroups per CU);
+  * bitmap_span_log2     rank window of the workgroup LDS bitmap (LDS size
+                          and so workgroups per CU);
+  * wave_window_log2     rank window of the per-wave LDS bitmap used for
+                          source lists of at most 64 entries;
   * hash_max_log2        longest source list given an LDS hash table.
 Each candidate must return the same counts as the default parameters.
 """
@@ -29,14 +31,16 @@ def tune_text(p):
             "merge_wave_min_log2 %s\n"
             "hash_max_log2 %d\n"
             "bitmap_span_log2 %d\n"
+            "wave_window_log2 %d\n"
             "fixed_skew_log2 %d\n"
             "fixed_hub_log2 %d\n" % (" ".join(str(x) for x in p["merge_wave_min_log2"]),
                                      p["hash_max_log2"], p["bitmap_span_log2"],
-                                     p["fixed_skew_log2"], p["fixed_hub_log2"]))
+                                     p["wave_window_log2"], p["fixed_skew_log2"],
+                                     p["fixed_hub_log2"]))
 
 
-DEFAULT = {"merge_wave_min_log2": [5] * 16, "hash_max_log2": 10, "bitmap_span_log2": 18,
-           "fixed_skew_log2": 2, "fixed_hub_log2": 9}
+DEFAULT = {"merge_wave_min_log2": [5] * 16, "hash_max_log2": 10, "bitmap_span_log2": 17,
+           "wave_window_log2": 15, "fixed_skew_log2": 2, "fixed_hub_log2": 9}
 
 
 def tuning_graphs():
@@ -89,7 +93,14 @@ def main():
         res[s] = total(p, "bitmap")
         print("bitmap_span_log2=%d: %.3f ms" % (s, res[s]), flush=True)
     best["bitmap_span_log2"] = min(res, key=res.get)
-    # 3. hash table cap
+    # 3. per-wave bitmap window
+    res = {}
+    for w in (13, 14, 15, 16):
+        p = dict(best, wave_window_log2=w)
+        res[w] = total(p, "bitmap")
+        print("wave_window_log2=%d: %.3f ms" % (w, res[w]), flush=True)
+    best["wave_window_log2"] = min(res, key=res.get)
+    # 4. hash table cap
     res = {}
     for h in (9, 10, 11):
         p = dict(best, hash_max_log2=h)

@@ -59,10 +59,18 @@ int hip_fail(stc_ctx *c, hipError_t e, const char *where) {
   return fail(c, STC_E_HIP, m);
 }
 
+// Column words are (rank << 1) | neg with 0xFFFFFFFF as padding, and the
+// bitmap probe relies on 2^32 - 2 rank exceeding twice the largest rank window
+// (2^19): both hold for ranks below 2^31 - 2^20.
+constexpr uint64_t kMaxVertices = (1ull << 31) - (1ull << 20);
+// Edge indices plus a load group (256) or a workgroup stride stay below 2^32.
+constexpr uint64_t kMaxLines = (1ull << 32) - (1ull << 20);
+
 void default_tune(TuneParams &t) {
   for (int i = 0; i < kBucketSide; i++) t.merge_wave_min_log2[i] = 5;
   t.hash_max_log2 = 10;
-  t.bitmap_span_log2 = 18;
+  t.bitmap_span_log2 = 17;
+  t.wave_window_log2 = 15;
   t.hash_block = 256;
   t.fixed_skew_log2 = 2;
   t.fixed_hub_log2 = 9;
@@ -105,6 +113,9 @@ bool parse_tune(std::istream &f, TuneParams &t) {
       if (!(is >> x.hash_max_log2) || x.hash_max_log2 < 4 || x.hash_max_log2 > 12) return false;
     } else if (k == "bitmap_span_log2") {
       if (!(is >> x.bitmap_span_log2) || x.bitmap_span_log2 < 10 || x.bitmap_span_log2 > 19)
+        return false;
+    } else if (k == "wave_window_log2") {
+      if (!(is >> x.wave_window_log2) || x.wave_window_log2 < 10 || x.wave_window_log2 > 16)
         return false;
     } else if (k == "fixed_skew_log2") {
       if (!(is >> x.fixed_skew_log2)) return false;
@@ -225,8 +236,8 @@ float median(std::vector<float> v) {
 int check_graph(stc_ctx *c, const stc_edges *g) {
   if (!g) return fail(c, STC_E_ARG, "null graph");
   if (g->m && (!g->u || !g->v || !g->sign)) return fail(c, STC_E_ARG, "null edge arrays");
-  if (g->n > (1u << 31)) return fail(c, STC_E_TOO_LARGE, "n must be <= 2^31");
-  if (g->m >= (1ull << 32)) return fail(c, STC_E_TOO_LARGE, "at most 2^32-1 lines");
+  if (g->n > kMaxVertices) return fail(c, STC_E_TOO_LARGE, "n must be <= 2^31 - 2^20");
+  if (g->m > kMaxLines) return fail(c, STC_E_TOO_LARGE, "at most 2^32 - 2^20 lines");
   return STC_OK;
 }
 
@@ -401,11 +412,13 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   c->stats.edges_merge = res.edges_by_strat[S_MERGE_LANE] + res.edges_by_strat[S_MERGE_WAVE];
   c->stats.edges_hash = res.edges_by_strat[S_HASH];
   c->stats.edges_bitmap = res.edges_by_strat[S_BITMAP];
-  // kernel order: merge-lane, merge-wave, wave-hash, workgroup-hash, bitmap
+  // kernel order: merge-lane, merge-wave, wave-hash, workgroup-hash,
+  // wave-bitmap, workgroup-bitmap
   const uint64_t words_k[kKernels] = {
       res.words_by_strat[S_MERGE_LANE], res.words_by_strat[S_MERGE_WAVE],
       res.words_by_strat[S_COUNT], res.words_by_strat[S_HASH] - res.words_by_strat[S_COUNT],
-      res.words_by_strat[S_BITMAP]};
+      res.words_by_strat[S_COUNT + 1],
+      res.words_by_strat[S_BITMAP] - res.words_by_strat[S_COUNT + 1]};
   for (int k = 0; k < kKernels; k++) {
     c->stats.ms_kernel[k] = median(kt[k]);
     c->stats.bytes_kernel[k] = 4.0 * (double)words_k[k];
@@ -492,7 +505,7 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
     int rc = check_graph(c, &graphs[i]);
     if (rc) return rc;
     uint64_t vb = (uint64_t)vbase[i] + graphs[i].n;
-    if (vb >= (1ull << 31)) return fail(c, STC_E_TOO_LARGE, "batch has >= 2^31 vertices");
+    if (vb > kMaxVertices) return fail(c, STC_E_TOO_LARGE, "batch has > 2^31 - 2^20 vertices");
     vbase[i + 1] = (uint32_t)vb;
     ebase[i + 1] = ebase[i] + graphs[i].m;
     if (i > 0 && graphs[i].m &&
@@ -503,7 +516,7 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
   }
   const uint64_t m = ebase[k];
   const uint32_t n = vbase[k];
-  if (m >= (1ull << 32)) return fail(c, STC_E_TOO_LARGE, "batch has >= 2^32 lines");
+  if (m > kMaxLines) return fail(c, STC_E_TOO_LARGE, "batch has > 2^32 - 2^20 lines");
   c->prepared = false;
   hipError_t e = hipSuccess;
   const uint64_t mm = m ? m : 1;

@@ -32,6 +32,9 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // bitmap kernel takes source lists of kWaveMaxDeg+1 .. kBitmapMaxDeg entries.
 constexpr uint32_t kWaveMaxDeg = 64;
 constexpr uint32_t kBitmapMaxDeg = 2048;
+// Entries of a workgroup-bitmap source list that fall below its rank window
+// are kept as a sorted LDS array of at most kLowCap words.
+constexpr uint32_t kLowCap = 1024;
 constexpr int kVertexBlock = 256;   // threads per workgroup of the hash kernel
 constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kernel
 
@@ -40,14 +43,19 @@ struct Ctrl {
   uint32_t bucket_cnt[kBuckets];
   uint32_t done;
   uint32_t len[6];   // merge-lane list, merge-wave list, hash vertices (workgroup),
-                     // bitmap vertices, hash vertices (wave)
+                     // bitmap vertices (workgroup), hash vertices (wave), bitmap
+                     // vertices (wave)
   uint32_t work[8];  // dynamic work counters
   uint32_t pad[1];
-  uint32_t queue[4][8][32];          // XCD work queues: one counter per XCD,
+  uint32_t queue[5][8][32];          // XCD work queues (workgroup hash,
+                                     // workgroup bitmap, wave hash, batch,
+                                     // wave bitmap): one counter per XCD,
                                      // 128 bytes apart
   unsigned long long t[4];           // class counters
   unsigned long long strat_edges[8]; // oriented edges per strategy (slot 5:
-                                     // hash edges of wave-hashed sources)
+                                     // hash edges of sources with at most
+                                     // kWaveMaxDeg out-edges; slot 6: bitmap
+                                     // edges of such sources)
   unsigned long long strat_words[8]; // sum of d+(u) + d+(v), same slots
   uint8_t table[kBuckets];
 };
@@ -57,6 +65,14 @@ namespace {
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   return (1ull << __lane_id()) - 1ull;
+}
+
+// Wave-uniform read of p[i] (i uniform) through the constant address space:
+// a scalar load, counted on lgkmcnt, so it never waits behind the wave's
+// outstanding vector loads.  Only for arrays no kernel writes while counting.
+__device__ __forceinline__ uint32_t sload(const uint32_t *p, uint32_t i) {
+  typedef const __attribute__((address_space(4))) uint32_t cu32;
+  return ((const cu32 *)p)[i];
 }
 
 __device__ __forceinline__ int clog2(uint32_t x) {
@@ -135,6 +151,16 @@ __device__ uint8_t decide(int bk, uint32_t cnt, const DispatchParams &p) {
   }
 }
 
+// A source list of d entries spanning span ranks fits the bitmap kernels when
+// it is short enough for one wave (any span: per-wave window plus at most d
+// low entries), or when its entries below the workgroup window fit kLowCap
+// (at most d - 1 of them lie below the window, none when span <= window).
+__device__ __forceinline__ bool bitmap_fits(const TuneParams &t, uint32_t d, uint32_t span) {
+  if (d <= kWaveMaxDeg) return true;
+  if (d > kBitmapMaxDeg) return false;
+  return span <= (1u << t.bitmap_span_log2) || d - 1u <= kLowCap;
+}
+
 // Final strategy of one edge: the bin's choice, demoted when the source list
 // does not fit the LDS structure.  Used identically by binning and by the
 // vertex-grouped kernels so both agree edge by edge.
@@ -145,9 +171,7 @@ __device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table,
   if (da_eff == 0 || db == 0) return S_SKIP;
   int bk = bucket_of(da_eff, db);
   uint8_t s = table[bk];
-  if (s == S_BITMAP && (span_a > (1u << t.bitmap_span_log2) || dfull_a <= kWaveMaxDeg ||
-                        dfull_a > kBitmapMaxDeg))
-    s = S_HASH;
+  if (s == S_BITMAP && !bitmap_fits(t, dfull_a, span_a)) s = S_HASH;
   if (s == S_HASH && dfull_a > (1u << t.hash_max_log2)) s = merge_variant(bk, t);
   return s;
 }
@@ -283,8 +307,8 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
     s_len[0] = s_len[1] = 0;
   }
   __syncthreads();
-  uint32_t cnt_s[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
-  uint64_t words_s[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
+  uint32_t cnt_s[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
+  uint64_t words_s[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
   uint32_t *lane_out = lane_seg + (uint64_t)blockIdx.x * seg;
   uint32_t *wave_out = wave_seg + (uint64_t)blockIdx.x * seg;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -321,12 +345,16 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
       cnt_s[k] += hit;
       words_s[k] += hit ? words : 0u;
     }
-    const bool wave_hashed = valid && s == S_HASH && (words - db_of_words) <= kWaveMaxDeg;
+    const bool small = (words - db_of_words) <= kWaveMaxDeg;
+    const bool wave_hashed = valid && s == S_HASH && small;
     cnt_s[S_COUNT] += wave_hashed;
     words_s[S_COUNT] += wave_hashed ? words : 0u;
+    const bool wave_bitmapped = valid && s == S_BITMAP && small;
+    cnt_s[S_COUNT + 1] += wave_bitmapped;
+    words_s[S_COUNT + 1] += wave_bitmapped ? words : 0u;
   }
 #pragma unroll
-  for (int k = 0; k <= S_COUNT; k++) {
+  for (int k = 0; k <= S_COUNT + 1; k++) {
     uint64_t c = cnt_s[k], w = words_s[k];
     for (int d = 32; d > 0; d >>= 1) {
       c += __shfl_down(c, d, 64);
@@ -366,17 +394,19 @@ __global__ __launch_bounds__(256) void k_list_compact(const uint32_t *__restrict
 }
 
 // Vertices with hash / bitmap work, in ascending rank: flags, then lists.
-// Hash sources with at most wave_max out-neighbours go to the wave list.
+// Sources with at most wave_max out-neighbours go to the wave lists.
 __global__ void k_vflag_split(uint32_t *__restrict__ vflag, const uint32_t *__restrict__ off,
                               uint32_t n, uint32_t wave_max, uint32_t *__restrict__ fh,
-                              uint32_t *__restrict__ fb, uint32_t *__restrict__ fw) {
+                              uint32_t *__restrict__ fb, uint32_t *__restrict__ fw,
+                              uint32_t *__restrict__ fwb) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x) {
     uint32_t f = vflag[v];
     uint32_t small = (off[v + 1] - off[v]) <= wave_max;
     fh[v] = (f & 1u) & (small ^ 1u);
     fw[v] = (f & 1u) & small;
-    fb[v] = (f >> 1) & 1u;
+    fb[v] = ((f >> 1) & 1u) & (small ^ 1u);
+    fwb[v] = ((f >> 1) & 1u) & small;
     if (f) vflag[v] = 0;
   }
 }
@@ -386,18 +416,23 @@ __global__ void k_vlist_scatter(const uint32_t *__restrict__ fh,
                                 const uint32_t *__restrict__ fb,
                                 const uint32_t *__restrict__ pb,
                                 const uint32_t *__restrict__ fw,
-                                const uint32_t *__restrict__ pw, uint32_t n,
+                                const uint32_t *__restrict__ pw,
+                                const uint32_t *__restrict__ fwb,
+                                const uint32_t *__restrict__ pwb, uint32_t n,
                                 uint32_t *__restrict__ vh, uint32_t *__restrict__ vb,
-                                uint32_t *__restrict__ vw, Ctrl *ctrl) {
+                                uint32_t *__restrict__ vw, uint32_t *__restrict__ vwb,
+                                Ctrl *ctrl) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += gridDim.x * blockDim.x) {
     if (fh[v]) vh[ph[v]] = v;
     if (fb[v]) vb[pb[v]] = v;
     if (fw[v]) vw[pw[v]] = v;
+    if (fwb[v]) vwb[pwb[v]] = v;
     if (v == n - 1) {
       ctrl->len[2] = ph[v] + fh[v];
       ctrl->len[3] = pb[v] + fb[v];
       ctrl->len[4] = pw[v] + fw[v];
+      ctrl->len[5] = pwb[v] + fwb[v];
     }
   }
 }
@@ -611,100 +646,176 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 constexpr int kUnroll = 4;  // independent 64-element loads in flight per wave
+// Word 3 of a raw (stride 0) buffer descriptor on gfx9-family GPUs (gfx950):
+// 32-bit data format, range checked against the byte count in word 2.
+constexpr int kBufferWord3 = 0x00020000;
 
-// Stream N+(v) = col[pb, eb) against the LDS 2-bit bitmap of N+(u) (entry r =
-// bits 2r, 2r+1 for rank lo + r) and add the edge's triangles to acc.  With
-// d2 = y - 2 lo = 2 (rank - lo) + neg(v,x), the word is d2 >> 5, the shift is
-// d2 & 30 and the span test is d2 < 2 span; padding (kEmpty) always fails it.
-template <bool CLS>
-__device__ __forceinline__ void stream_bitmap(const uint32_t *__restrict__ col, uint32_t pb,
-                                              uint32_t eb, uint32_t sab,
-                                              const uint32_t *__restrict__ B, uint32_t lo2,
-                                              uint32_t span2, Acc<CLS> &acc) {
-  const uint32_t lane = __lane_id();
-  uint32_t e0 = 0, e1 = 0, e2 = 0;
-  for (uint32_t base = pb; base < eb; base += 64u * kUnroll) {
-    uint32_t y[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const uint32_t idx = base + u * 64u + lane;
-      y[u] = idx < eb ? col[idx] : kEmpty;
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      if (base + u * 64u >= eb) break;  // wave-uniform
-      const uint32_t d2 = y[u] - lo2;
-      uint32_t w = 0;
-      if (d2 < span2) w = B[d2 >> 5];
-      tally<CLS>((w >> (d2 & 30u)) & 3u, d2 & 1u, e0, e1, e2);
-    }
+// Look up rank yr among the nlow sorted words L[0, nlow) ((rank << 1) | neg)
+// by a branch-free lower bound: bit 0 of the result = found, bit 1 = neg.
+__device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, uint32_t nlow,
+                                               uint32_t yr) {
+  uint32_t pos = 0;  // entries below yr seen so far
+  for (uint32_t step = nlow ? 1u << (31 - __clz(nlow)) : 0u; step; step >>= 1)
+    if (pos + step <= nlow && (L[pos + step - 1] >> 1) < yr) pos += step;
+  if (pos < nlow) {
+    const uint32_t x = L[pos];
+    if ((x >> 1) == yr) return 1u | ((x & 1u) << 1);
   }
-  acc.add_edge(sab, e0, e1, e2);
+  return 0u;
 }
 
-// Same against an LDS hash table of N+(u) (slot = rank & mask, linear probe).
-template <bool CLS>
-__device__ __forceinline__ void stream_hash(const uint32_t *__restrict__ col, uint32_t pb,
-                                            uint32_t eb, uint32_t sab,
-                                            const uint32_t *__restrict__ T, uint32_t mask,
-                                            Acc<CLS> &acc) {
-  const uint32_t lane = __lane_id();
-  uint32_t e0 = 0, e1 = 0, e2 = 0;
-  for (uint32_t base = pb; base < eb; base += 64u * kUnroll) {
-    uint32_t y[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const uint32_t idx = base + u * 64u + lane;
-      y[u] = idx < eb ? col[idx] : kEmpty;
+// ---- probes of a staged source list -------------------------------------------
+// Each returns, per lane, bit 0 = target rank found in N+(u), bit 1 = neg(u,x).
+
+// LDS 2-bit bitmap B over the rank window [lo, lo + span) of N+(u) (entry r =
+// bits 2r, 2r+1 for rank lo + r) and the sorted array L[0, nlow) of the
+// entries below lo.  With d2 = y - 2 lo = 2 (rank - lo) + neg(v,x), the word is
+// d2 >> 5, the shift is d2 & 30 and the window test is d2 < 2 span; padding
+// (kEmpty) and ranks above the window fail it.  A 64-element group with a
+// target below lo (a prefix of the sorted N+(v), so rare) also searches L for
+// those lanes.  A target below lo wraps d2 to at least 2^32 - 2 lo > 2^21 >=
+// 2 span, and kEmpty - 2 lo exceeds 2 span too, because the API admits at most
+// 2^31 - 2^20 vertices (ranks).
+struct BitmapProbe {
+  const uint32_t *B;
+  uint32_t lo2, span2;
+  const uint32_t *L;
+  uint32_t nlow;
+  __device__ __forceinline__ uint32_t operator()(uint32_t y) const {
+    const uint32_t d2 = y - lo2;
+    uint32_t w = 0;
+    if (d2 < span2) w = B[d2 >> 5];
+    uint32_t q = (w >> (d2 & 30u)) & 3u;
+    if (__ballot(y < lo2)) {  // wave-uniform
+      if (y < lo2) q = low_lookup(L, nlow, y >> 1);
     }
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      if (base + u * 64u >= eb) break;  // wave-uniform
-      const uint32_t yr = y[u] >> 1;
-      uint32_t q = 0;
-      if (y[u] != kEmpty) {
-        uint32_t h = yr & mask;
-        while (true) {
-          const uint32_t x = T[h];
-          if (x == kEmpty) break;
-          if ((x >> 1) == yr) {
-            q = 1u | ((x & 1u) << 1);
-            break;
-          }
-          h = (h + 1) & mask;
+    return q;
+  }
+};
+
+// LDS hash table T of N+(u): slot = rank & mask, linear probing, kEmpty free.
+struct HashProbe {
+  const uint32_t *T;
+  uint32_t mask;
+  __device__ __forceinline__ uint32_t operator()(uint32_t y) const {
+    uint32_t q = 0;
+    if (y != kEmpty) {
+      const uint32_t yr = y >> 1;
+      uint32_t h = yr & mask;
+      while (true) {
+        const uint32_t x = T[h];
+        if (x == kEmpty) break;
+        if ((x >> 1) == yr) {
+          q = 1u | ((x & 1u) << 1);
+          break;
         }
+        h = (h + 1) & mask;
       }
-      tally<CLS>(q, y[u] & 1u, e0, e1, e2);
     }
+    return q;
   }
-  acc.add_edge(sab, e0, e1, e2);
-}
+};
 
-// Walk the out-edges j = first, first+step, ... of source a (ending at ea),
-// prefetching the next edge's target offsets while the current one streams.
-template <class Body>
-__device__ __forceinline__ void walk_edges(const DevCsr &g, uint32_t first, uint32_t ea,
-                                           uint32_t step, const Body &body) {
-  uint32_t j = __builtin_amdgcn_readfirstlane(first);
-  uint32_t w = 0, pb = 0, eb = 0;
-  if (j < ea) {
-    w = __builtin_amdgcn_readfirstlane(g.col[j]);
-    pb = __builtin_amdgcn_readfirstlane(g.off[w >> 1]);
-    eb = __builtin_amdgcn_readfirstlane(g.off[(w >> 1) + 1]);
-  }
-  while (j < ea) {
-    const uint32_t jn = j + step;
-    uint32_t wn = 0, pbn = 0, ebn = 0;
-    if (jn < ea) {
-      wn = __builtin_amdgcn_readfirstlane(g.col[jn]);
-      pbn = __builtin_amdgcn_readfirstlane(g.off[wn >> 1]);
-      ebn = __builtin_amdgcn_readfirstlane(g.off[(wn >> 1) + 1]);
+// Stream the out-lists N+(v) of the source's out-edges j = first, first +
+// step, ... (below ea) that `keep` accepts against `probe`, adding each
+// edge's triangles to acc.
+//   Edge metadata moves 64 edges at a time, one edge per lane: the column
+// word, the target's list bounds and the keep flag are loaded and evaluated
+// in parallel, and the kept edges are visited through the ballot with
+// readlane, so no memory latency sits between consecutive edges.
+//   List entries move in units of up to 64 * kUnroll entries of one list.  The
+// next unit's loads (the rest of the list, or the head of the next kept
+// list) are issued before the current unit is probed, so a wave keeps two
+// units of loads in flight across edge boundaries.  A unit is read with
+// buffer loads through a descriptor whose base is the unit's first entry and
+// whose range ends at min(list end, unit end): lanes past the list end read 0
+// without touching memory and are masked to kEmpty when the unit is probed.
+// The loads are unconditional, their per-lane offsets never change and only
+// the descriptor (SGPRs) moves, so each wait counts exactly the loads it
+// needs and no address register is rewritten under a load in flight.
+template <bool CLS, class Keep, class Probe>
+__device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, uint32_t ea,
+                                             uint32_t step, const Keep &keep,
+                                             const Probe &probe, Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  constexpr uint32_t kUnit = 64u * kUnroll;
+  auto load = [&](uint32_t(&y)[kUnroll], uint32_t pos, uint32_t eb) {
+    const uint32_t nrec = pos < eb ? min(eb - pos, kUnit) * 4u : 0u;
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(col + pos), (short)0, (int)nrec, kBufferWord3);
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++)
+      y[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)((u * 64u + lane) * 4u), 0, 0);
+  };
+  uint32_t e0 = 0, e1 = 0, e2 = 0;
+  // a unit: column word of its edge (bit 0 = sign), first entry, list end
+  struct Unit {
+    uint32_t w, pos, eb;
+  };
+  auto lane_of = [](uint32_t v, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)v, i); };
+  auto probe_unit = [&](const uint32_t(&y)[kUnroll], const Unit &c) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      if (c.pos + u * 64u >= c.eb) break;  // wave-uniform
+      const uint32_t yu = c.pos + u * 64u + lane < c.eb ? y[u] : kEmpty;
+      tally<CLS>(probe(yu), yu & 1u, e0, e1, e2);
     }
-    body(j, w, pb, eb);
-    j = jn;
-    w = wn;
-    pb = pbn;
-    eb = ebn;
+    // Retire the buffer's loads here (a no-op use), so a group skipped past
+    // the list end leaves nothing pending into the loop's next turn.
+    asm volatile("" ::"v"(y[0]), "v"(y[1]), "v"(y[2]), "v"(y[3]));
+    if (c.pos + kUnit >= c.eb) {  // last unit of its list
+      acc.add_edge(c.w & 1u, e0, e1, e2);
+      e0 = e1 = e2 = 0;
+    }
+  };
+  for (uint32_t j0 = first; j0 < ea; j0 += 64u * step) {
+    const uint32_t j = j0 + lane * step;
+    uint32_t w = 0, pb = 0, eb = 0;
+    if (j < ea) {
+      w = col[j];
+      pb = g.off[w >> 1];
+      eb = g.off[(w >> 1) + 1];
+    }
+    uint64_t mask = __ballot(j < ea && keep(j, pb, eb));
+    // next unit after c: the rest of its list, else the next kept edge's list;
+    // when none is left, n repeats c past its end (its loads stay in bounds)
+    auto advance = [&](const Unit &c, Unit &n) -> bool {
+      if (c.pos + kUnit < c.eb) {
+        n = Unit{c.w, c.pos + kUnit, c.eb};
+        return true;
+      }
+      if (mask) {
+        const int i = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        n = Unit{lane_of(w, i), lane_of(pb, i), lane_of(eb, i)};
+        return true;
+      }
+      n = Unit{c.w, c.pos + kUnit, c.eb};
+      return false;
+    };
+    if (!mask) continue;
+    const int i0 = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    Unit a{lane_of(w, i0), lane_of(pb, i0), lane_of(eb, i0)};
+    uint32_t ya[kUnroll], yb[kUnroll];
+    load(ya, a.pos, a.eb);
+    // two buffers in turn: the other buffer's loads are in flight while one
+    // is probed
+    while (true) {
+      Unit b;
+      const bool vb = advance(a, b);
+      load(yb, b.pos, b.eb);
+      probe_unit(ya, a);
+      if (!vb) break;
+      const bool va = advance(b, a);
+      load(ya, a.pos, a.eb);
+      probe_unit(yb, b);
+      if (!va) break;
+    }
+    // the last turn's loads past the end: retire them before the next batch
+    asm volatile("" ::"v"(ya[0]), "v"(ya[1]), "v"(ya[2]), "v"(ya[3]));
+    asm volatile("" ::"v"(yb[0]), "v"(yb[1]), "v"(yb[2]), "v"(yb[3]));
   }
 }
 
@@ -738,10 +849,12 @@ __device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, u
     while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
   }
   __syncthreads();
-  walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
-    if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
-    stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
-  });
+  stream_edges<CLS>(
+      g, la + wave, ea, nwaves,
+      [&](uint32_t j, uint32_t pb, uint32_t eb) {
+        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_HASH;
+      },
+      HashProbe{T, mask}, acc);
   __syncthreads();
 }
 
@@ -749,10 +862,12 @@ __device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, u
 // kWaveCap slots and counts a's hash edges.
 constexpr uint32_t kWaveCap = 4 * kWaveMaxDeg;
 
+// `want` names the strategy whose edges this call counts: S_HASH, or S_BITMAP
+// where a caller has no per-wave bitmap (the batch kernel).
 template <bool CLS>
 __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, uint32_t *T,
                                                  const uint8_t *table, const TuneParams &tune,
-                                                 Acc<CLS> &acc) {
+                                                 Acc<CLS> &acc, uint8_t want = S_HASH) {
   const uint32_t *__restrict__ col = g.col;
   const uint32_t lane = __lane_id();
   const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
@@ -768,19 +883,30 @@ __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, ui
     while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
   }
   wave_lds_sync();
-  walk_edges(g, la, ea, 1u, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
-    if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_HASH) return;
-    stream_hash<CLS>(col, pb, eb, w & 1u, T, mask, acc);
-  });
+  stream_edges<CLS>(
+      g, la, ea, 1u,
+      [&](uint32_t j, uint32_t pb, uint32_t eb) {
+        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == want;
+      },
+      HashProbe{T, mask}, acc);
   wave_lds_sync();
 }
 
-// Stage N+(a) as a 2-bit (present, negative) LDS bitmap over its rank span:
-// entry r (bits 2r, 2r+1) holds rank lo + r, so a probe is one LDS read.  B
-// must be zero on entry and is zero again on return (only the words the
+// Rank window of a source list [lo, hi] for a bitmap of `window` entries: the
+// top of the list, where the targets' out-lists concentrate (high ranks are
+// high degrees).  Returns the window base.
+__device__ __forceinline__ uint32_t window_base(uint32_t lo, uint32_t hi, uint32_t window) {
+  return hi - lo + 1u > window ? hi - window + 1u : lo;
+}
+
+// Stage N+(a) as a 2-bit (present, negative) LDS bitmap over the rank window
+// [base, hi] (entry r = bits 2r, 2r+1 for rank base + r), with the entries
+// below base (a prefix of the sorted list, at most kLowCap of them) copied to
+// L.  B must be zero on entry and is zero again on return (only the words the
 // vertex touched are cleared).  Called by every thread of the workgroup.
 template <bool CLS>
 __device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a, uint32_t *B,
+                                                    uint32_t *L, uint32_t *s_nlow,
                                                     const uint8_t *table,
                                                     const TuneParams &tune, Acc<CLS> &acc) {
   const uint32_t *__restrict__ col = g.col;
@@ -789,20 +915,73 @@ __device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a,
   const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
   const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
   const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
-  const uint32_t span = span_of(col, la, ea);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(col[ea - 1] >> 1);
+  const uint32_t span = hi - lo + 1u;
+  const uint32_t base = window_base(lo, hi, 1u << tune.bitmap_span_log2);
   for (uint32_t k = tid; k < d; k += nthreads) {
     const uint32_t x = col[la + k];
-    const uint32_t r = (x >> 1) - lo;
-    atomicOr(&B[r >> 4], (1u | ((x & 1u) << 1)) << ((r & 15u) * 2u));
+    const uint32_t r = x >> 1;
+    if (r >= base) {
+      const uint32_t o = r - base;
+      atomicOr(&B[o >> 4], (1u | ((x & 1u) << 1)) << ((o & 15u) * 2u));
+      if (k == 0) *s_nlow = 0u;
+    } else {
+      L[k] = x;
+      if ((col[la + k + 1] >> 1) >= base) *s_nlow = k + 1u;  // last low entry (hi >= base)
+    }
   }
   __syncthreads();
-  walk_edges(g, la + wave, ea, nwaves, [&](uint32_t j, uint32_t w, uint32_t pb, uint32_t eb) {
-    if (edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) != S_BITMAP) return;
-    stream_bitmap<CLS>(col, pb, eb, w & 1u, B, lo << 1, span << 1, acc);
-  });
+  const uint32_t nlow = __builtin_amdgcn_readfirstlane(*s_nlow);
+  stream_edges<CLS>(
+      g, la + wave, ea, nwaves,
+      [&](uint32_t j, uint32_t pb, uint32_t eb) {
+        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_BITMAP;
+      },
+      BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow}, acc);
   __syncthreads();
-  for (uint32_t k = tid; k < d; k += nthreads) B[((col[la + k] >> 1) - lo) >> 4] = 0u;
+  for (uint32_t k = tid; k < d; k += nthreads) {
+    const uint32_t r = col[la + k] >> 1;
+    if (r >= base) B[(r - base) >> 4] = 0u;
+  }
   __syncthreads();
+}
+
+// One wave stages N+(a) (at most kWaveMaxDeg entries) as a 2-bit bitmap over
+// a rank window of `window` entries in its own region B, the entries below
+// the window in L (kWaveMaxDeg words), and counts a's bitmap edges.
+template <bool CLS>
+__device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, uint32_t *B,
+                                                   uint32_t *L, uint32_t window,
+                                                   const uint8_t *table,
+                                                   const TuneParams &tune, Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
+  const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(col[ea - 1] >> 1);
+  const uint32_t span = hi - lo + 1u;
+  const uint32_t base = window_base(lo, hi, window);
+  const bool mine = lane < d;
+  uint32_t x = 0, o = 0;
+  if (mine) {
+    x = col[la + lane];
+    o = (x >> 1) - base;  // wraps for entries below the window
+  }
+  const bool in_w = mine && (x >> 1) >= base;
+  if (in_w) atomicOr(&B[o >> 4], (1u | ((x & 1u) << 1)) << ((o & 15u) * 2u));
+  else if (mine) L[lane] = x;  // the low entries are the lanes [0, nlow)
+  const uint32_t nlow = (uint32_t)__popcll(__ballot(mine && !in_w));
+  wave_lds_sync();
+  stream_edges<CLS>(
+      g, la, ea, 1u,
+      [&](uint32_t j, uint32_t pb, uint32_t eb) {
+        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_BITMAP;
+      },
+      BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow}, acc);
+  wave_lds_sync();
+  if (in_w) B[o >> 4] = 0u;
+  wave_lds_sync();
 }
 
 // Pop one item for the whole workgroup from an XCD queue.
@@ -873,15 +1052,17 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   acc.flush(ctrl);
 }
 
-// One workgroup per source vertex: LDS bitmap (zeroed once per workgroup).
+// One workgroup per source vertex: LDS bitmap (zeroed once per workgroup)
+// followed by the kLowCap-word low array.
 template <bool CLS>
 __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                          const DispatchParams *P,
                                                          const uint32_t *__restrict__ vlist,
                                                          uint32_t bitmap_words) {
   extern __shared__ uint32_t B[];
+  uint32_t *L = B + bitmap_words;
   __shared__ uint8_t table[kBuckets];
-  __shared__ uint32_t s_first, s_count;
+  __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
   if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
@@ -891,16 +1072,53 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
   Acc<CLS> acc;
   uint32_t first;
   while (block_pop(ctrl->queue[1], len, cur, &s_first, &s_count, first))
-    bitmap_vertex_block<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first]), B, table, tune,
-                             acc);
+    bitmap_vertex_block<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first]), B, L, &s_nlow,
+                             table, tune, acc);
+  acc.flush(ctrl);
+}
+
+// One wave per source vertex (source lists of at most kWaveMaxDeg entries):
+// each wave owns window_words bitmap words followed by kWaveMaxDeg low words.
+template <bool CLS>
+__global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
+                                                     const DispatchParams *P,
+                                                     const uint32_t *__restrict__ vlist,
+                                                     uint32_t window_words) {
+  extern __shared__ uint32_t lds[];
+  __shared__ uint8_t table[kBuckets];
+  __shared__ TuneParams tune;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = __lane_id();
+  table[threadIdx.x] = ctrl->table[threadIdx.x];
+  if (threadIdx.x == 0) tune = P->tune;
+  uint32_t *B = lds + wave * (window_words + kWaveMaxDeg);
+  uint32_t *L = B + window_words;
+  for (uint32_t k = lane; k < window_words; k += 64) B[k] = 0u;
+  __syncthreads();
+  const uint32_t window = window_words * 16u;
+  const uint32_t len = ctrl->len[5];
+  QueueCursor cur{blockIdx.x & 7u, 0u};
+  Acc<CLS> acc;
+  while (true) {
+    uint32_t first = 0, count = 0;
+    if (lane == 0 && !queue_pop(ctrl->queue[4], len, kWaveChunk, cur, first, count)) count = 0;
+    first = __shfl(first, 0, 64);
+    count = __shfl(count, 0, 64);
+    if (count == 0) break;
+    for (uint32_t it = 0; it < count; it++)
+      bitmap_vertex_wave<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first + it]), B, L,
+                              window, table, tune, acc);
+  }
   acc.flush(ctrl);
 }
 
 // ---- batched counting: one persistent kernel over the union of k graphs ----
 // Items, in queue order (popped from the top, so heaviest kinds first):
 //   merge-lane chunks of kLaneItem edges (one per thread), merge-wave chunks
-//   of one edge per wave, wave-hash chunks of one vertex per wave, hash
-//   vertices, bitmap vertices.  Every item's triangles are added to the
+//   of one edge per wave, wave-hash and wave-bitmap chunks of one vertex per
+//   wave (both counted through the wave's hash table: a per-wave bitmap
+//   window would cost the workgroup bitmap its LDS), hash vertices, bitmap
+//   vertices.  Every item's triangles are added to the
 //   counters of the graph owning its source vertex (graph_of is indexed by
 //   rank; each graph's ranks are one contiguous range).
 constexpr uint32_t kBatchBlock = 256;
@@ -930,14 +1148,15 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
     DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vh,
     const uint32_t *__restrict__ vb, const uint32_t *__restrict__ vw,
-    const uint32_t *__restrict__ graph_of, unsigned long long *__restrict__ per_graph,
-    uint32_t bitmap_words, uint32_t hash_slots) {
+    const uint32_t *__restrict__ vwb, const uint32_t *__restrict__ graph_of,
+    unsigned long long *__restrict__ per_graph, uint32_t bitmap_words, uint32_t hash_slots) {
   extern __shared__ uint32_t lds[];
   uint32_t *B = lds;                                  // bitmap, kept zero between items
-  uint32_t *T = lds + bitmap_words;                   // hash table (hash_slots)
+  uint32_t *L = B + bitmap_words;                     // low array (kLowCap)
+  uint32_t *T = L + kLowCap;                          // hash table (hash_slots)
   uint32_t *TW = T + hash_slots;                      // per-wave tables
   __shared__ uint8_t table[kBuckets];
-  __shared__ uint32_t s_first, s_count;
+  __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
   const uint32_t tid = threadIdx.x, lane = __lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -945,11 +1164,13 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
   if (tid == 0) tune = P->tune;
   for (uint32_t k = tid; k < bitmap_words; k += blockDim.x) B[k] = 0u;
   const uint32_t nl = ctrl->len[0], nwv = ctrl->len[1], nh = ctrl->len[2], nb = ctrl->len[3],
-                 nhw = ctrl->len[4];
+                 nhw = ctrl->len[4], nbw = ctrl->len[5];
   const uint32_t i_lane = (nl + kLaneItem - 1) / kLaneItem;
   const uint32_t i_wave = (nwv + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
   const uint32_t i_hw = (nhw + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
-  const uint32_t b1 = i_lane, b2 = b1 + i_wave, b3 = b2 + i_hw, b4 = b3 + nh, total = b4 + nb;
+  const uint32_t i_bw = (nbw + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
+  const uint32_t b1 = i_lane, b2 = b1 + i_wave, b3 = b2 + i_hw, b3w = b3 + i_bw, b4 = b3w + nh,
+                 total = b4 + nb;
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<false> acc;
   uint32_t it;
@@ -976,16 +1197,23 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
       const uint32_t i = (it - b2) * kWavesPerBatchBlock + wave;
       if (i < nhw) {
         const uint32_t a = __builtin_amdgcn_readfirstlane(vw[i]);
-        hash_vertex_wave<false>(g, a, TW + wave * kWaveCap, table, tune, acc);
+        hash_vertex_wave<false>(g, a, TW + wave * kWaveCap, table, tune, acc, S_HASH);
+        acc.flush_graph(per_graph, graph_of[a]);
+      }
+    } else if (it < b3w) {  // wave-bitmap chunk, counted by hash probes
+      const uint32_t i = (it - b3) * kWavesPerBatchBlock + wave;
+      if (i < nbw) {
+        const uint32_t a = __builtin_amdgcn_readfirstlane(vwb[i]);
+        hash_vertex_wave<false>(g, a, TW + wave * kWaveCap, table, tune, acc, S_BITMAP);
         acc.flush_graph(per_graph, graph_of[a]);
       }
     } else if (it < b4) {  // hash vertex
-      const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b3]);
+      const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b3w]);
       hash_vertex_block<false>(g, a, T, table, tune, acc);
       acc.flush_graph(per_graph, graph_of[a]);
     } else {  // bitmap vertex
       const uint32_t a = __builtin_amdgcn_readfirstlane(vb[it - b4]);
-      bitmap_vertex_block<false>(g, a, B, table, tune, acc);
+      bitmap_vertex_block<false>(g, a, B, L, &s_nlow, table, tune, acc);
       acc.flush_graph(per_graph, graph_of[a]);
     }
   }
@@ -1010,7 +1238,10 @@ size_t hash_lds_bytes(const TuneParams &t) {
   return (size_t)4u << std::max(t.hash_max_log2 + 2, 6);
 }
 size_t bitmap_lds_bytes(const TuneParams &t) {
-  return (size_t)((1u << t.bitmap_span_log2) / 4u);
+  return (size_t)((1u << t.bitmap_span_log2) / 4u) + 4u * kLowCap;
+}
+size_t wave_bitmap_lds_bytes(const TuneParams &t) {
+  return 4u * (size_t)(256 / 64) * ((1u << t.wave_window_log2) / 16u + kWaveMaxDeg);
 }
 
 #define STC_TRY(expr)                \
@@ -1024,8 +1255,9 @@ size_t bitmap_lds_bytes(const TuneParams &t) {
 hipError_t init_kernel_attributes() {
   static bool done = false;
   if (done) return hipSuccess;
-  const void *fns[] = {(const void *)k_bitmap<true>, (const void *)k_bitmap<false>,
-                       (const void *)k_hash<true>, (const void *)k_hash<false>,
+  const void *fns[] = {(const void *)k_bitmap<true>,      (const void *)k_bitmap<false>,
+                       (const void *)k_bitmap_wave<true>, (const void *)k_bitmap_wave<false>,
+                       (const void *)k_hash<true>,        (const void *)k_hash<false>,
                        (const void *)k_batch};
   for (const void *f : fns)
     STC_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
@@ -1036,12 +1268,12 @@ hipError_t init_kernel_attributes() {
 namespace {
 
 struct BinOut {
-  uint32_t *lane_list, *wave_list, *vh, *vb, *vw;
+  uint32_t *lane_list, *wave_list, *vh, *vb, *vw, *vwb;
 };
 
 // Buffers: cb.gcount holds Ctrl; cb.lists the per-workgroup segments and the
 // packed merge-lane / merge-wave edge lists; cb.vflag n words kept zero
-// between runs; cb.vlist 9n words (flags, their scans and the three vertex
+// between runs; cb.vlist 12n words (flags, their scans and the four vertex
 // lists).  Every buffer is sized before the first launch, so a second call
 // with the same graph allocates nothing (and can be captured in a graph).
 hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
@@ -1062,7 +1294,7 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
                        0, st, cb.vflag.as<uint32_t>(), cb.vflag.bytes / 4, 0u);
     STC_TRY(hipGetLastError());
   }
-  STC_TRY(cb.vlist.ensure(9 * ((uint64_t)n + 1) * 4));
+  STC_TRY(cb.vlist.ensure(12 * ((uint64_t)n + 1) * 4));
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
   uint32_t *lane_seg = cb.lists.as<uint32_t>();
   uint32_t *wave_seg = lane_seg + seg_words;
@@ -1077,6 +1309,8 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   out.vb = out.vh + (n + 1);
   uint32_t *fw = out.vb + (n + 1), *pw = fw + (n + 1);
   out.vw = pw + (n + 1);
+  uint32_t *fwb = out.vw + (n + 1), *pwb = fwb + (n + 1);
+  out.vwb = pwb + (n + 1);
 
   if (marks) STC_TRY(hipEventRecord(marks[0], st));
   hipLaunchKernelGGL(k_fill_words, dim3(8), dim3(256), 0, st, (uint32_t *)ctrl,
@@ -1095,13 +1329,14 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   if (n > 0) {
     const int vgrid = grid_for(n, 256, num_cus * 8);
     hipLaunchKernelGGL(k_vflag_split, dim3(vgrid), dim3(256), 0, st, vf, g.off, n,
-                       kWaveMaxDeg, fh, fb, fw);
+                       kWaveMaxDeg, fh, fb, fw, fwb);
     STC_TRY(hipGetLastError());
     STC_TRY(scan_exclusive_u32(fh, ph, n, cb.scan, st));
     STC_TRY(scan_exclusive_u32(fb, pb, n, cb.scan, st));
     STC_TRY(scan_exclusive_u32(fw, pw, n, cb.scan, st));
+    STC_TRY(scan_exclusive_u32(fwb, pwb, n, cb.scan, st));
     hipLaunchKernelGGL(k_vlist_scatter, dim3(vgrid), dim3(256), 0, st, fh, ph, fb, pb, fw,
-                       pw, n, out.vh, out.vb, out.vw, ctrl);
+                       pw, fwb, pwb, n, out.vh, out.vb, out.vw, out.vwb, ctrl);
     STC_TRY(hipGetLastError());
   }
   if (marks) STC_TRY(hipEventRecord(marks[1], st));
@@ -1120,8 +1355,11 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
   const size_t hash_lds = hash_lds_bytes(hp.tune);
   const size_t bm_lds = bitmap_lds_bytes(hp.tune);
   const int hash_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (128 * 1024) / hash_lds));
-  const int bm_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (144 * 1024) / bm_lds));
+  const int bm_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (148 * 1024) / bm_lds));
   const uint32_t bm_words = (1u << hp.tune.bitmap_span_log2) / 16u;
+  const size_t bw_lds = wave_bitmap_lds_bytes(hp.tune);
+  const int bw_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (152 * 1024) / bw_lds));
+  const uint32_t bw_words = (1u << hp.tune.wave_window_log2) / 16u;
   auto mark = [&](int i) -> hipError_t {
     return marks ? hipEventRecord(marks[i], st) : hipSuccess;
   };
@@ -1135,9 +1373,12 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL(k_hash<true>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
                        dp, L.vh);
     STC_TRY(mark(5));
+    hipLaunchKernelGGL(k_bitmap_wave<true>, dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl, dp,
+                       L.vwb, bw_words);
+    STC_TRY(mark(6));
     hipLaunchKernelGGL(k_bitmap<true>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
                        dp, L.vb, bm_words);
-    STC_TRY(mark(6));
+    STC_TRY(mark(7));
   } else {
     hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, L.lane_list);
     STC_TRY(mark(2));
@@ -1148,16 +1389,19 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL(k_hash<false>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
                        dp, L.vh);
     STC_TRY(mark(5));
+    hipLaunchKernelGGL(k_bitmap_wave<false>, dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl, dp,
+                       L.vwb, bw_words);
+    STC_TRY(mark(6));
     hipLaunchKernelGGL(k_bitmap<false>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
                        dp, L.vb, bm_words);
-    STC_TRY(mark(6));
+    STC_TRY(mark(7));
   }
   return hipGetLastError();
 }
 
 size_t batch_lds_bytes(const TuneParams &t) {
-  return 4u * ((size_t)((1u << t.bitmap_span_log2) / 16u) + ((size_t)4u << t.hash_max_log2) +
-               kWavesPerBatchBlock * kWaveCap);
+  return 4u * ((size_t)((1u << t.bitmap_span_log2) / 16u) + kLowCap +
+               ((size_t)4u << t.hash_max_log2) + kWavesPerBatchBlock * kWaveCap);
 }
 
 hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
@@ -1173,7 +1417,7 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
   const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (150 * 1024) / lds));
   hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
                      cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vb, L.vw,
-                     graph_of, per_graph, (1u << hp.tune.bitmap_span_log2) / 16u,
+                     L.vwb, graph_of, per_graph, (1u << hp.tune.bitmap_span_log2) / 16u,
                      4u << hp.tune.hash_max_log2);
   return hipGetLastError();
 }
@@ -1184,7 +1428,7 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_
   STC_TRY(hipStreamSynchronize(st));
   for (int k = 0; k < 4; k++) r.t[k] = h.t[k];
   (void)classes;
-  for (int k = 0; k <= S_COUNT; k++) {
+  for (int k = 0; k <= S_COUNT + 1; k++) {
     r.edges_by_strat[k] = h.strat_edges[k];
     r.words_by_strat[k] = h.strat_words[k];
   }

@@ -20,7 +20,8 @@ enum Strategy : uint8_t {
   S_MERGE_LANE = 1,  // one lane per edge, two-pointer merge
   S_MERGE_WAVE = 2,  // one wave64 per edge, LDS-tiled cooperative merge
   S_HASH = 3,        // LDS hash table of the source list
-  S_BITMAP = 4,      // LDS presence/sign bitmap over the source list's span
+  S_BITMAP = 4,      // LDS presence/sign bitmap over a rank window at the top
+                     // of the source list (lower part kept as a sorted array)
   S_COUNT = 5
 };
 
@@ -58,7 +59,8 @@ struct TuneParams {
   // lane per edge, else one wave per edge
   int merge_wave_min_log2[kBucketSide];  // indexed by log2 of shorter side
   int hash_max_log2;      // hash only if the source list has <= 2^x entries
-  int bitmap_span_log2;   // bitmap span capacity (entries)
+  int bitmap_span_log2;   // rank window of the workgroup bitmap (entries)
+  int wave_window_log2;   // rank window of the per-wave bitmap (entries)
   int hash_block;         // threads per workgroup for hash/bitmap
   int fixed_skew_log2;    // fixed rule: skew >= 2^x -> hash
   int fixed_hub_log2;     // fixed rule: d+(v) >= 2^x (or source) -> bitmap
@@ -164,19 +166,21 @@ struct CountBuffers {
 
 struct CountResult {
   uint64_t t[4] = {0, 0, 0, 0};  // t0..t3 when classes, else t[0]=bal t[1]=unbal
-  // slots S_SKIP..S_BITMAP, then S_COUNT: hash edges of wave-hashed sources
-  uint64_t edges_by_strat[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
-  uint64_t words_by_strat[S_COUNT + 1] = {0, 0, 0, 0, 0, 0};
+  // slots S_SKIP..S_BITMAP, then S_COUNT: hash edges of sources counted one
+  // wave per vertex, S_COUNT + 1: bitmap edges of such sources
+  uint64_t edges_by_strat[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
+  uint64_t words_by_strat[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
   uint8_t table[kBuckets];
   uint32_t bucket_cnt[kBuckets];
 };
 
 // Enqueue binning + counting on `st` (no host sync inside).  hp is the host
-// copy of *dp (launch shapes come from its tune part).  marks (nullable, 7
+// copy of *dp (launch shapes come from its tune part).  marks (nullable, 8
 // events) are recorded before binning, after binning and after each of the
-// merge-lane, merge-wave, wave-hash, workgroup-hash and bitmap kernels.
-constexpr int kMarks = 7;
-constexpr int kKernels = 5;
+// merge-lane, merge-wave, wave-hash, workgroup-hash, wave-bitmap and
+// workgroup-bitmap kernels.
+constexpr int kMarks = 8;
+constexpr int kKernels = 6;
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
                          int num_cus, hipStream_t st, const hipEvent_t *marks);
@@ -185,6 +189,7 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r,
                       hipStream_t st);
 size_t hash_lds_bytes(const TuneParams &t);
 size_t bitmap_lds_bytes(const TuneParams &t);
+size_t wave_bitmap_lds_bytes(const TuneParams &t);
 
 // Batched counting over the union CSR of k graphs: binning, then one
 // persistent kernel; per_graph gets k (balanced, unbalanced) pairs.
