@@ -138,7 +138,9 @@ int stc_count_prepared(stc_ctx *ctx, const stc_opts *opts, stc_counts *out);
 int stc_count_batch(stc_ctx *ctx, const stc_edges *graphs, uint32_t k,
                     const stc_opts *opts, stc_counts *out);
 
-int stc_get_stats(const stc_ctx *ctx, stc_stats *out);
+/* Statistics of the last counting call.  The per-kernel edge and byte counts
+ * are computed by one extra device pass on the first call after a count. */
+int stc_get_stats(stc_ctx *ctx, stc_stats *out);
 
 /* Reference operator shape (signed_tc.hxx:527-542): counts one graph and
  * writes the two totals; returns elapsed milliseconds of the counting phase,

@@ -36,6 +36,7 @@ struct stc_ctx {
   BuildResult br;
   DevCsr g;
   bool prepared = false;
+  bool kstats_pending = false;  // per-kernel statistics of the last run not read yet
   stc_stats stats{};
   stc_classes classes{};
   bool have_classes = false;
@@ -351,6 +352,7 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   c->g.src = c->bb.src.as<uint32_t>();
   c->prepared = true;
   std::memset(&c->stats, 0, sizeof(c->stats));
+  c->kstats_pending = false;
   c->stats.n = c->br.n;
   c->stats.m_in = c->br.m_in;
   c->stats.m_oriented = c->br.m;
@@ -408,21 +410,8 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
   c->stats.ms_bin = bins.back();
-  c->stats.edges_skipped = res.edges_by_strat[S_SKIP];
-  c->stats.edges_merge = res.edges_by_strat[S_MERGE_LANE] + res.edges_by_strat[S_MERGE_WAVE];
-  c->stats.edges_hash = res.edges_by_strat[S_HASH];
-  c->stats.edges_bitmap = res.edges_by_strat[S_BITMAP];
-  // kernel order: merge-lane, merge-wave, wave-hash, workgroup-hash,
-  // wave-bitmap, workgroup-bitmap
-  const uint64_t words_k[kKernels] = {
-      res.words_by_strat[S_MERGE_LANE], res.words_by_strat[S_MERGE_WAVE],
-      res.words_by_strat[S_COUNT], res.words_by_strat[S_HASH] - res.words_by_strat[S_COUNT],
-      res.words_by_strat[S_COUNT + 1],
-      res.words_by_strat[S_BITMAP] - res.words_by_strat[S_COUNT + 1]};
-  for (int k = 0; k < kKernels; k++) {
-    c->stats.ms_kernel[k] = median(kt[k]);
-    c->stats.bytes_kernel[k] = 4.0 * (double)words_k[k];
-  }
+  for (int k = 0; k < kKernels; k++) c->stats.ms_kernel[k] = median(kt[k]);
+  c->kstats_pending = true;
   c->stats.dispatch_used = (mode == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : mode;
   if (res_out) *res_out = res;
   return STC_OK;
@@ -461,8 +450,25 @@ int stc_last_classes(const stc_ctx *c, stc_classes *out) {
   return STC_OK;
 }
 
-int stc_get_stats(const stc_ctx *c, stc_stats *out) {
+int stc_get_stats(stc_ctx *c, stc_stats *out) {
   if (!c || !out) return STC_E_ARG;
+  if (c->kstats_pending) {
+    // Per-kernel edge / word counts of the last run, by one pass over its
+    // bucket bytes and strategy table (kept out of the timed counting path).
+    (void)hipSetDevice(c->device);
+    CountResult res;
+    hipError_t e = enqueue_stats(c->g, c->dp, c->cb, c->num_cus, c->st);
+    if (e == hipSuccess) e = read_count(c->cb, false, res, c->st);
+    if (e != hipSuccess) return hip_fail(c, e, "count statistics");
+    // kernel order: merge-lane, merge-wave, wave-hash, workgroup-hash,
+    // wave-bitmap, workgroup-bitmap; slot 7 counts skipped edges
+    c->stats.edges_skipped = res.kern_edges[7];
+    c->stats.edges_merge = res.kern_edges[0] + res.kern_edges[1];
+    c->stats.edges_hash = res.kern_edges[2] + res.kern_edges[3];
+    c->stats.edges_bitmap = res.kern_edges[4] + res.kern_edges[5];
+    for (int k = 0; k < kKernels; k++) c->stats.bytes_kernel[k] = 4.0 * (double)res.kern_words[k];
+    c->kstats_pending = false;
+  }
   *out = c->stats;
   return STC_OK;
 }
@@ -579,6 +585,7 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
   c->g.src = c->bb.src.as<uint32_t>();
   c->prepared = true;
   std::memset(&c->stats, 0, sizeof(c->stats));
+  c->kstats_pending = false;
   c->stats.n = c->br.n;
   c->stats.m_in = c->br.m_in;
   c->stats.m_oriented = c->br.m;
@@ -635,6 +642,7 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
     out[i].unbalanced = got[2 * i + 1];
   }
   c->stats.ms_count = median(times);
+  c->kstats_pending = true;
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
   c->stats.dispatch_used = (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
   c->have_classes = false;

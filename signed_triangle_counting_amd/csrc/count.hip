@@ -52,11 +52,11 @@ struct Ctrl {
                                      // wave bitmap): one counter per XCD,
                                      // 128 bytes apart
   unsigned long long t[4];           // class counters
-  unsigned long long strat_edges[8]; // oriented edges per strategy (slot 5:
-                                     // hash edges of sources with at most
-                                     // kWaveMaxDeg out-edges; slot 6: bitmap
-                                     // edges of such sources)
-  unsigned long long strat_words[8]; // sum of d+(u) + d+(v), same slots
+  unsigned long long kern_edges[8];  // oriented edges counted per kernel
+                                     // (merge-lane, merge-wave, wave-hash,
+                                     // workgroup-hash, wave-bitmap,
+                                     // workgroup-bitmap); slot 7: skipped
+  unsigned long long kern_words[8];  // sum of d+(u) + d+(v) over those edges
   uint8_t table[kBuckets];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl alignment");
@@ -161,20 +161,28 @@ __device__ __forceinline__ bool bitmap_fits(const TuneParams &t, uint32_t d, uin
   return span <= (1u << t.bitmap_span_log2) || d - 1u <= kLowCap;
 }
 
-// Final strategy of one edge: the bin's choice, demoted when the source list
-// does not fit the LDS structure.  Used identically by binning and by the
-// vertex-grouped kernels so both agree edge by edge.
-__device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table,
-                                                 const TuneParams &t,
-                                                 uint32_t dfull_a, uint32_t span_a,
-                                                 uint32_t da_eff, uint32_t db) {
-  if (da_eff == 0 || db == 0) return S_SKIP;
-  int bk = bucket_of(da_eff, db);
+// Final strategy of an edge in bucket bk: the bin's choice, demoted when the
+// source list (dfull_a entries over span_a ranks) does not fit the LDS
+// structure.  Binning (from the stored bucket) and the vertex-grouped kernels
+// (through edge_strategy) apply the same rule, so both agree edge by edge.
+__device__ __forceinline__ uint8_t bucket_strategy(const uint8_t *table, const TuneParams &t,
+                                                   uint32_t dfull_a, uint32_t span_a, int bk) {
   uint8_t s = table[bk];
   if (s == S_BITMAP && !bitmap_fits(t, dfull_a, span_a)) s = S_HASH;
   if (s == S_HASH && dfull_a > (1u << t.hash_max_log2)) s = merge_variant(bk, t);
   return s;
 }
+
+__device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table,
+                                                 const TuneParams &t,
+                                                 uint32_t dfull_a, uint32_t span_a,
+                                                 uint32_t da_eff, uint32_t db) {
+  if (da_eff == 0 || db == 0) return S_SKIP;
+  return bucket_strategy(table, t, dfull_a, span_a, bucket_of(da_eff, db));
+}
+
+// Bucket byte of an edge that cannot close a triangle (da = 0 or db = 0).
+constexpr uint8_t kNoBucket = 0xFF;
 
 __device__ __forceinline__ uint32_t span_of(const uint32_t *__restrict__ col,
                                             uint32_t la, uint32_t ea) {
@@ -242,8 +250,11 @@ __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uin
 }
 
 // ---- binning ----------------------------------------------------------------
+// Bucket histogram; every edge's bucket byte goes to bk8 (kNoBucket when
+// the edge cannot close a triangle) so the scatter pass need not recompute it.
 __global__ __launch_bounds__(256) void k_bin_hist(DevCsr g, Ctrl *ctrl,
-                                                  const DispatchParams *P) {
+                                                  const DispatchParams *P,
+                                                  uint8_t *__restrict__ bk8) {
   __shared__ uint32_t h[kBuckets];
   __shared__ int last;
   const int tid = threadIdx.x;
@@ -252,14 +263,16 @@ __global__ __launch_bounds__(256) void k_bin_hist(DevCsr g, Ctrl *ctrl,
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
-    bool valid = e < g.m;
+    const bool in = e < g.m;
+    bool valid = false;
     uint32_t bk = 0;
-    if (valid) {
+    if (in) {
       uint32_t a = g.src[e], b = g.col[e] >> 1;
       uint32_t da = g.off[a + 1] - (uint32_t)e - 1u;
       uint32_t db = g.off[b + 1] - g.off[b];
       valid = da && db;
       if (valid) bk = (uint32_t)bucket_of(da, db);
+      bk8[e] = valid ? (uint8_t)bk : kNoBucket;
     }
     uint32_t rank, cnt;
     wave_match8(bk, valid, rank, cnt);
@@ -290,8 +303,14 @@ __device__ __forceinline__ uint32_t lds_append(bool pred, uint32_t *s_len) {
   return base + (uint32_t)__popcll(mask & lanemask_lt());
 }
 
+// Strategy of every edge from its stored bucket byte; merge edges go to the
+// workgroup's list segments, hash and bitmap edges flag their source vertex
+// (bit 0 hash, bit 1 bitmap).  Edges are sorted by source, so the lanes of one
+// source form runs: each run's first lane sets the run's flag bits with one
+// atomic, only when they are not set yet.
 __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
+                                                     const uint8_t *__restrict__ bk8,
                                                      uint32_t *__restrict__ lane_seg,
                                                      uint32_t *__restrict__ wave_seg,
                                                      uint32_t seg,
@@ -301,67 +320,44 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
   __shared__ TuneParams tune;
   __shared__ uint32_t s_len[2];
   const int tid = threadIdx.x;
+  const uint32_t lane = __lane_id();
   table[tid] = ctrl->table[tid];
   if (tid == 0) {
     tune = P->tune;
     s_len[0] = s_len[1] = 0;
   }
   __syncthreads();
-  uint32_t cnt_s[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
-  uint64_t words_s[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
   uint32_t *lane_out = lane_seg + (uint64_t)blockIdx.x * seg;
   uint32_t *wave_out = wave_seg + (uint64_t)blockIdx.x * seg;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
     uint8_t s = S_SKIP;
-    uint32_t a = 0, words = 0, db_of_words = 0;
-    const bool valid = e < g.m;
-    if (valid) {
+    uint32_t a = 0xFFFFFFFFu;
+    if (e < g.m) {
       a = g.src[e];
-      uint32_t b = g.col[e] >> 1;
-      uint32_t la = g.off[a], ea = g.off[a + 1];
-      uint32_t da = ea - (uint32_t)e - 1u;
-      uint32_t db = g.off[b + 1] - g.off[b];
-      words = (ea - la) + db;
-      db_of_words = db;
-      if (da && db) {
-        uint8_t s0 = table[bucket_of(da, db)];
-        uint32_t span = s0 == S_BITMAP ? span_of(g.col, la, ea) : 0u;
-        s = edge_strategy(table, tune, ea - la, span, da, db);
+      const uint32_t bk = bk8[e];
+      if (bk != kNoBucket) {
+        const uint32_t la = g.off[a], ea = g.off[a + 1];
+        const uint32_t span = table[bk] == S_BITMAP ? span_of(g.col, la, ea) : 0u;
+        s = bucket_strategy(table, tune, ea - la, span, (int)bk);
       }
     }
     uint32_t p = lds_append(s == S_MERGE_LANE, &s_len[0]);
     if (s == S_MERGE_LANE) lane_out[p] = (uint32_t)e;
     p = lds_append(s == S_MERGE_WAVE, &s_len[1]);
     if (s == S_MERGE_WAVE) wave_out[p] = (uint32_t)e;
-    if (s == S_HASH || s == S_BITMAP) {
-      uint32_t bit = s == S_HASH ? 1u : 2u;
-      if (!(vflag[a] & bit)) atomicOr(&vflag[a], bit);
+    // runs of equal sources: heads, and each head's run as a lane mask
+    const uint32_t prev = (uint32_t)__shfl_up((int)a, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || prev != a);
+    const uint64_t hb = __ballot(s == S_HASH), bb = __ballot(s == S_BITMAP);
+    if ((heads >> lane) & 1u) {
+      const uint64_t above = heads & ~((2ull << lane) - 1ull);
+      const uint64_t run = (above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull) &
+                           ~((1ull << lane) - 1ull);
+      const uint32_t bits = ((hb & run) ? 1u : 0u) | ((bb & run) ? 2u : 0u);
+      if (bits && (vflag[a] & bits) != bits) atomicOr(&vflag[a], bits);
     }
-#pragma unroll
-    for (int k = 0; k < S_COUNT; k++) {
-      const bool hit = valid && s == k;
-      cnt_s[k] += hit;
-      words_s[k] += hit ? words : 0u;
-    }
-    const bool small = (words - db_of_words) <= kWaveMaxDeg;
-    const bool wave_hashed = valid && s == S_HASH && small;
-    cnt_s[S_COUNT] += wave_hashed;
-    words_s[S_COUNT] += wave_hashed ? words : 0u;
-    const bool wave_bitmapped = valid && s == S_BITMAP && small;
-    cnt_s[S_COUNT + 1] += wave_bitmapped;
-    words_s[S_COUNT + 1] += wave_bitmapped ? words : 0u;
-  }
-#pragma unroll
-  for (int k = 0; k <= S_COUNT + 1; k++) {
-    uint64_t c = cnt_s[k], w = words_s[k];
-    for (int d = 32; d > 0; d >>= 1) {
-      c += __shfl_down(c, d, 64);
-      w += __shfl_down(w, d, 64);
-    }
-    wave_add(&ctrl->strat_edges[k], c);
-    wave_add(&ctrl->strat_words[k], w);
   }
   __syncthreads();
   if (tid == 0) {
@@ -434,6 +430,47 @@ __global__ void k_vlist_scatter(const uint32_t *__restrict__ fh,
       ctrl->len[4] = pw[v] + fw[v];
       ctrl->len[5] = pwb[v] + fwb[v];
     }
+  }
+}
+
+// Per-kernel statistics of the last counting run, outside the timed path:
+// every edge's kernel by the same rule binning applies (bucket byte, source
+// list fit, source length for wave or workgroup kernels), with its oriented
+// edge count and d+(u) + d+(v) sum.  Slots follow the launch order
+// (merge-lane, merge-wave, wave-hash, workgroup-hash, wave-bitmap,
+// workgroup-bitmap); slot 7 counts the edges that cannot close a triangle.
+__global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
+                                                    const DispatchParams *P,
+                                                    const uint8_t *__restrict__ bk8) {
+  __shared__ uint8_t table[kBuckets];
+  __shared__ TuneParams tune;
+  __shared__ unsigned long long s_e[8], s_w[8];
+  const int tid = threadIdx.x;
+  table[tid] = ctrl->table[tid];
+  if (tid == 0) tune = P->tune;
+  if (tid < 8) s_e[tid] = s_w[tid] = 0;
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < g.m; e += stride) {
+    const uint32_t bk = bk8[e];
+    uint32_t slot = 7, words = 0;
+    if (bk != kNoBucket) {
+      const uint32_t a = g.src[e], b = g.col[e] >> 1;
+      const uint32_t la = g.off[a], ea = g.off[a + 1];
+      const uint32_t span = table[bk] == S_BITMAP ? span_of(g.col, la, ea) : 0u;
+      const uint8_t s = bucket_strategy(table, tune, ea - la, span, (int)bk);
+      const bool small = ea - la <= kWaveMaxDeg;
+      slot = s == S_MERGE_LANE ? 0u : s == S_MERGE_WAVE ? 1u : s == S_HASH ? (small ? 2u : 3u)
+                                                                        : (small ? 4u : 5u);
+      words = (ea - la) + (g.off[b + 1] - g.off[b]);
+    }
+    atomicAdd(&s_e[slot], 1ull);
+    atomicAdd(&s_w[slot], (unsigned long long)words);
+  }
+  __syncthreads();
+  if (tid < 8) {
+    if (s_e[tid]) atomicAdd(&ctrl->kern_edges[tid], s_e[tid]);
+    if (s_w[tid]) atomicAdd(&ctrl->kern_words[tid], s_w[tid]);
   }
 }
 
@@ -645,7 +682,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kUnroll = 4;  // independent 64-element loads in flight per wave
+constexpr int kUnroll = 8;  // 64-element loads per unit (a wave keeps two units in flight)
 // Word 3 of a raw (stride 0) buffer descriptor on gfx9-family GPUs (gfx950):
 // 32-bit data format, range checked against the byte count in word 2.
 constexpr int kBufferWord3 = 0x00020000;
@@ -763,7 +800,8 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
     }
     // Retire the buffer's loads here (a no-op use), so a group skipped past
     // the list end leaves nothing pending into the loop's next turn.
-    asm volatile("" ::"v"(y[0]), "v"(y[1]), "v"(y[2]), "v"(y[3]));
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(y[u]));
     if (c.pos + kUnit >= c.eb) {  // last unit of its list
       acc.add_edge(c.w & 1u, e0, e1, e2);
       e0 = e1 = e2 = 0;
@@ -814,8 +852,8 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
       if (!va) break;
     }
     // the last turn's loads past the end: retire them before the next batch
-    asm volatile("" ::"v"(ya[0]), "v"(ya[1]), "v"(ya[2]), "v"(ya[3]));
-    asm volatile("" ::"v"(yb[0]), "v"(yb[1]), "v"(yb[2]), "v"(yb[3]));
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(ya[u]), "v"(yb[u]));
   }
 }
 
@@ -1288,6 +1326,7 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   const uint64_t seg_words = (uint64_t)seg * bin_grid;
   STC_TRY(cb.lists.ensure((2 * seg_words + 2 * (m ? m : 1)) * 4));
   STC_TRY(cb.blk.ensure((4 * (uint64_t)bin_grid + 8) * 4));
+  STC_TRY(cb.bk8.ensure(m ? m : 1));
   if (cb.vflag.bytes < ((uint64_t)n + 1) * 4) {
     STC_TRY(cb.vflag.ensure(((uint64_t)n + 1) * 4));
     hipLaunchKernelGGL(k_fill_words, dim3(grid_for((uint64_t)n + 1, 256, 8192)), dim3(256),
@@ -1316,10 +1355,11 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   hipLaunchKernelGGL(k_fill_words, dim3(8), dim3(256), 0, st, (uint32_t *)ctrl,
                      (uint64_t)(sizeof(Ctrl) / 4), 0u);
   STC_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_bin_hist, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp);
+  hipLaunchKernelGGL(k_bin_hist, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp,
+                     cb.bk8.as<uint8_t>());
   STC_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_bin_scatter, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp, lane_seg,
-                     wave_seg, seg, blk_cnt, vf);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp,
+                     cb.bk8.as<uint8_t>(), lane_seg, wave_seg, seg, blk_cnt, vf);
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(blk_cnt, blk_pos, 2 * (uint64_t)bin_grid, cb.scan, st));
   hipLaunchKernelGGL(k_list_compact, dim3(2 * bin_grid), dim3(256), 0, st, lane_seg, wave_seg,
@@ -1399,6 +1439,13 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
   return hipGetLastError();
 }
 
+hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
+                         int num_cus, hipStream_t st) {
+  hipLaunchKernelGGL(k_edge_stats, dim3(grid_for(g.m, 256, num_cus * 8)), dim3(256), 0, st, g,
+                     cb.gcount.as<Ctrl>(), dp, cb.bk8.as<uint8_t>());
+  return hipGetLastError();
+}
+
 size_t batch_lds_bytes(const TuneParams &t) {
   return 4u * ((size_t)((1u << t.bitmap_span_log2) / 16u) + kLowCap +
                ((size_t)4u << t.hash_max_log2) + kWavesPerBatchBlock * kWaveCap);
@@ -1428,9 +1475,9 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_
   STC_TRY(hipStreamSynchronize(st));
   for (int k = 0; k < 4; k++) r.t[k] = h.t[k];
   (void)classes;
-  for (int k = 0; k <= S_COUNT + 1; k++) {
-    r.edges_by_strat[k] = h.strat_edges[k];
-    r.words_by_strat[k] = h.strat_words[k];
+  for (int k = 0; k < 8; k++) {
+    r.kern_edges[k] = h.kern_edges[k];
+    r.kern_words[k] = h.kern_words[k];
   }
   for (int k = 0; k < kBuckets; k++) {
     r.table[k] = h.table[k];

@@ -160,16 +160,17 @@ hipError_t shift_batch_ids(uint32_t *u, uint32_t *v, uint64_t m, const uint64_t 
                            hipStream_t st);
 
 struct CountBuffers {
-  DevBuf gcount, lists, blk, vflag, vlist;
+  DevBuf gcount, lists, blk, vflag, vlist, bk8;
   ScanScratch scan;
 };
 
 struct CountResult {
   uint64_t t[4] = {0, 0, 0, 0};  // t0..t3 when classes, else t[0]=bal t[1]=unbal
-  // slots S_SKIP..S_BITMAP, then S_COUNT: hash edges of sources counted one
-  // wave per vertex, S_COUNT + 1: bitmap edges of such sources
-  uint64_t edges_by_strat[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
-  uint64_t words_by_strat[S_COUNT + 2] = {0, 0, 0, 0, 0, 0, 0};
+  // per counting kernel (merge-lane, merge-wave, wave-hash, workgroup-hash,
+  // wave-bitmap, workgroup-bitmap; slot 7: skipped edges): oriented edges and
+  // their sum of d+(u) + d+(v)
+  uint64_t kern_edges[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t kern_words[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint8_t table[kBuckets];
   uint32_t bucket_cnt[kBuckets];
 };
@@ -184,7 +185,11 @@ constexpr int kKernels = 6;
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
                          int num_cus, hipStream_t st, const hipEvent_t *marks);
-// Read back totals and per-strategy sizes after the stream completes.
+// Per-kernel edge and word counts of the run just enqueued (not timed: enqueue
+// after the run's closing event).
+hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
+                         int num_cus, hipStream_t st);
+// Read back totals and per-kernel sizes after the stream completes.
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r,
                       hipStream_t st);
 size_t hash_lds_bytes(const TuneParams &t);

@@ -285,3 +285,33 @@ def test_partial_tables_strategy_invariant(ctx, scale, ef):
         c = ctx.count_with_table(t)
         total = (total[0] + c[0], total[1] + c[1])
     assert total == oracle_counts(g.n, g.u, g.v, g.sign)
+
+
+def test_kernel_statistics_account_for_every_edge(ctx):
+    """Per-kernel edge counts (from the counting kernels) plus skipped edges
+    (from binning) add up to the oriented edge count in every mode, and the
+    per-kernel algorithmic bytes add up to the B_alg of the counted edges."""
+    g = stc.gen_rmat(17, 16, 0.57, 0.19, 0.19, 0.25, 4242)
+    o = oracle.Csr(g.n, g.u, g.v, g.sign)
+    off, E, _, R = o.arrays()
+    n = o.n
+    fd = np.diff(off.astype(np.int64))
+    # oriented edges in rank space, each list ascending by rank
+    key = np.sort(R[np.repeat(np.arange(n), fd)].astype(np.int64) * n + R[E])
+    srcr, dstr = key // n, key % n
+    fdr = np.zeros(n, np.int64)
+    fdr[R] = fd
+    offr = np.concatenate([[0], np.cumsum(fdr)])
+    da = fdr[srcr] - (np.arange(len(key)) - offr[srcr]) - 1  # entries after the target
+    counted = (da > 0) & (fdr[dstr] > 0)
+    words = int((fdr[srcr] + fdr[dstr])[counted].sum())
+    ctx.prepare(g)
+    for mode in MODES:
+        ctx.count_prepared(dispatch=mode)
+        st = ctx.stats()
+        say("stats", mode, st["edges_merge"], st["edges_hash"], st["edges_bitmap"],
+            st["edges_skipped"])
+        assert (st["edges_merge"] + st["edges_hash"] + st["edges_bitmap"] +
+                st["edges_skipped"]) == st["m_oriented"] == o.m
+        assert st["edges_skipped"] == int((~counted).sum())
+        assert sum(st["bytes_kernel"]) == 4 * words
