@@ -226,7 +226,8 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst, uint64_t v) {
 // the XCD and a range's vertices share that XCD's L2).  Each range has its own
 // counter on its own 128-byte line and is drained from its end (highest ranks,
 // the heaviest vertices, first); a drained range sends the caller on to the
-// next XCD's range.  Returns false when every range is empty.
+// next XCD's range.  Returns false when every range is empty.  (Kernels read
+// their list length first and exit at once when it is zero.)
 struct QueueCursor {
   uint32_t q, tried;
 };
@@ -389,47 +390,99 @@ __global__ __launch_bounds__(256) void k_list_compact(const uint32_t *__restrict
     ctrl->len[1] = blk_pos[2 * nblk - 1] + cnt - blk_pos[nblk];
 }
 
-// Vertices with hash / bitmap work, in ascending rank: flags, then lists.
-// Sources with at most wave_max out-neighbours go to the wave lists.
-__global__ void k_vflag_split(uint32_t *__restrict__ vflag, const uint32_t *__restrict__ off,
-                              uint32_t n, uint32_t wave_max, uint32_t *__restrict__ fh,
-                              uint32_t *__restrict__ fb, uint32_t *__restrict__ fw,
-                              uint32_t *__restrict__ fwb) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += gridDim.x * blockDim.x) {
-    uint32_t f = vflag[v];
-    uint32_t small = (off[v + 1] - off[v]) <= wave_max;
-    fh[v] = (f & 1u) & (small ^ 1u);
-    fw[v] = (f & 1u) & small;
-    fb[v] = ((f >> 1) & 1u) & (small ^ 1u);
-    fwb[v] = ((f >> 1) & 1u) & small;
-    if (f) vflag[v] = 0;
+// Vertices with hash / bitmap work go to four lists in ascending rank:
+// workgroup hash, workgroup bitmap, wave hash, wave bitmap (sources with at
+// most kWaveMaxDeg out-edges use the wave kernels).  A workgroup owns a tile
+// of kVTile consecutive vertices, visited in rounds of 256: k_vcount counts
+// each list's members per tile, an exclusive scan over the tile counts gives
+// each tile's list offsets, and k_vscatter writes the members in order and
+// clears the flags.
+constexpr uint32_t kVTile = 4096;
+
+__device__ __forceinline__ void vertex_lists_of(uint32_t v, uint32_t n, const uint32_t *vflag,
+                                                const uint32_t *off, bool c[4], uint32_t &f) {
+  f = 0;
+  bool small = false;
+  if (v < n) {
+    f = vflag[v];
+    if (f) small = off[v + 1] - off[v] <= kWaveMaxDeg;
   }
+  c[0] = (f & 1u) && !small;
+  c[1] = (f & 2u) && !small;
+  c[2] = (f & 1u) && small;
+  c[3] = (f & 2u) && small;
 }
 
-__global__ void k_vlist_scatter(const uint32_t *__restrict__ fh,
-                                const uint32_t *__restrict__ ph,
-                                const uint32_t *__restrict__ fb,
-                                const uint32_t *__restrict__ pb,
-                                const uint32_t *__restrict__ fw,
-                                const uint32_t *__restrict__ pw,
-                                const uint32_t *__restrict__ fwb,
-                                const uint32_t *__restrict__ pwb, uint32_t n,
-                                uint32_t *__restrict__ vh, uint32_t *__restrict__ vb,
-                                uint32_t *__restrict__ vw, uint32_t *__restrict__ vwb,
-                                Ctrl *ctrl) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += gridDim.x * blockDim.x) {
-    if (fh[v]) vh[ph[v]] = v;
-    if (fb[v]) vb[pb[v]] = v;
-    if (fw[v]) vw[pw[v]] = v;
-    if (fwb[v]) vwb[pwb[v]] = v;
-    if (v == n - 1) {
-      ctrl->len[2] = ph[v] + fh[v];
-      ctrl->len[3] = pb[v] + fb[v];
-      ctrl->len[4] = pw[v] + fw[v];
-      ctrl->len[5] = pwb[v] + fwb[v];
+__global__ __launch_bounds__(256) void k_vcount(const uint32_t *__restrict__ vflag,
+                                                const uint32_t *__restrict__ off, uint32_t n,
+                                                uint32_t *__restrict__ tile_cnt) {
+  __shared__ uint32_t s_cnt[4];
+  const uint32_t tid = threadIdx.x, ntiles = gridDim.x;
+  if (tid < 4) s_cnt[tid] = 0;
+  __syncthreads();
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  const uint32_t base = blockIdx.x * kVTile;
+  for (uint32_t r = 0; r < kVTile; r += 256) {
+    bool c[4];
+    uint32_t f;
+    vertex_lists_of(base + r + tid, n, vflag, off, c, f);
+#pragma unroll
+    for (int L = 0; L < 4; L++) cnt[L] += (uint32_t)__popcll(__ballot(c[L]));
+  }
+  if (__lane_id() == 0)
+#pragma unroll
+    for (int L = 0; L < 4; L++) atomicAdd(&s_cnt[L], cnt[L]);
+  __syncthreads();
+  if (tid < 4) tile_cnt[tid * ntiles + blockIdx.x] = s_cnt[tid];
+}
+
+// tile_pos: exclusive scan of tile_cnt (list-major, 4 * ntiles entries).
+__global__ __launch_bounds__(256) void k_vscatter(uint32_t *__restrict__ vflag,
+                                                  const uint32_t *__restrict__ off, uint32_t n,
+                                                  const uint32_t *__restrict__ tile_cnt,
+                                                  const uint32_t *__restrict__ tile_pos,
+                                                  uint32_t *__restrict__ vh,
+                                                  uint32_t *__restrict__ vb,
+                                                  uint32_t *__restrict__ vw,
+                                                  uint32_t *__restrict__ vwb, Ctrl *ctrl) {
+  __shared__ uint32_t s_wcnt[4][4];  // [list][wave] members in this round
+  __shared__ uint32_t s_run[4];      // members of earlier rounds in this tile
+  const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  const uint32_t ntiles = gridDim.x, b = blockIdx.x;
+  uint32_t *const lists[4] = {vh, vb, vw, vwb};
+  uint32_t tile_off[4];
+#pragma unroll
+  for (int L = 0; L < 4; L++) tile_off[L] = tile_pos[L * ntiles + b] - tile_pos[L * ntiles];
+  if (tid < 4) s_run[tid] = 0;
+  const uint32_t base = b * kVTile;
+  for (uint32_t r = 0; r < kVTile; r += 256) {
+    const uint32_t v = base + r + tid;
+    bool c[4];
+    uint32_t f;
+    vertex_lists_of(v, n, vflag, off, c, f);
+    uint64_t m[4];
+#pragma unroll
+    for (int L = 0; L < 4; L++) {
+      m[L] = __ballot(c[L]);
+      if (lane == 0) s_wcnt[L][wave] = (uint32_t)__popcll(m[L]);
     }
+    __syncthreads();
+#pragma unroll
+    for (int L = 0; L < 4; L++) {
+      if (c[L]) {
+        uint32_t p = tile_off[L] + s_run[L] + (uint32_t)__popcll(m[L] & lanemask_lt());
+        for (uint32_t w = 0; w < wave; w++) p += s_wcnt[L][w];
+        lists[L][p] = v;
+      }
+    }
+    if (f) vflag[v] = 0u;
+    __syncthreads();
+    if (tid < 4) s_run[tid] += s_wcnt[tid][0] + s_wcnt[tid][1] + s_wcnt[tid][2] + s_wcnt[tid][3];
+    __syncthreads();
+  }
+  if (b == ntiles - 1 && tid < 4) {
+    const uint32_t last = tid * ntiles + ntiles - 1;
+    ctrl->len[2 + tid] = tile_pos[last] + tile_cnt[last] - tile_pos[tid * ntiles];
   }
 }
 
@@ -1047,9 +1100,10 @@ __global__ __launch_bounds__(kVertexBlock) void k_hash(DevCsr g, Ctrl *ctrl,
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
+  const uint32_t len = ctrl->len[2];
+  if (len == 0) return;  // the whole grid: no vertex has workgroup-hash edges
   if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
-  const uint32_t len = ctrl->len[2];
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   uint32_t first;
@@ -1068,13 +1122,14 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   __shared__ uint32_t Tall[4 * kWaveCap];
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
+  const uint32_t len = ctrl->len[4];
+  if (len == 0) return;  // the whole grid: no vertex has wave-hash edges
   const int wave = threadIdx.x >> 6;
   const uint32_t lane = __lane_id();
   table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
   __syncthreads();
   uint32_t *T = Tall + wave * kWaveCap;
-  const uint32_t len = ctrl->len[4];
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   while (true) {
@@ -1102,10 +1157,11 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
+  const uint32_t len = ctrl->len[3];
+  if (len == 0) return;  // the whole grid: no vertex has workgroup-bitmap edges
   if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
   for (uint32_t k = threadIdx.x; k < bitmap_words; k += blockDim.x) B[k] = 0u;
-  const uint32_t len = ctrl->len[3];
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   uint32_t first;
@@ -1125,6 +1181,8 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   extern __shared__ uint32_t lds[];
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
+  const uint32_t len = ctrl->len[5];
+  if (len == 0) return;  // the whole grid: no vertex has wave-bitmap edges
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = __lane_id();
   table[threadIdx.x] = ctrl->table[threadIdx.x];
@@ -1134,7 +1192,6 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   for (uint32_t k = lane; k < window_words; k += 64) B[k] = 0u;
   __syncthreads();
   const uint32_t window = window_words * 16u;
-  const uint32_t len = ctrl->len[5];
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   while (true) {
@@ -1311,8 +1368,8 @@ struct BinOut {
 
 // Buffers: cb.gcount holds Ctrl; cb.lists the per-workgroup segments and the
 // packed merge-lane / merge-wave edge lists; cb.vflag n words kept zero
-// between runs; cb.vlist 12n words (flags, their scans and the four vertex
-// lists).  Every buffer is sized before the first launch, so a second call
+// between runs; cb.vlist the four vertex lists (n + 1 words each) and the
+// per-tile list counts and offsets.  Every buffer is sized before the first launch, so a second call
 // with the same graph allocates nothing (and can be captured in a graph).
 hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
                            int num_cus, hipStream_t st, const hipEvent_t *marks, BinOut &out) {
@@ -1333,7 +1390,8 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
                        0, st, cb.vflag.as<uint32_t>(), cb.vflag.bytes / 4, 0u);
     STC_TRY(hipGetLastError());
   }
-  STC_TRY(cb.vlist.ensure(12 * ((uint64_t)n + 1) * 4));
+  const uint32_t ntiles = (uint32_t)(((uint64_t)n + kVTile - 1) / kVTile);
+  STC_TRY(cb.vlist.ensure((4 * ((uint64_t)n + 1) + 8 * (uint64_t)ntiles + 8) * 4));
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
   uint32_t *lane_seg = cb.lists.as<uint32_t>();
   uint32_t *wave_seg = lane_seg + seg_words;
@@ -1342,14 +1400,12 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   uint32_t *blk_cnt = cb.blk.as<uint32_t>();
   uint32_t *blk_pos = blk_cnt + 2 * bin_grid;
   uint32_t *vf = cb.vflag.as<uint32_t>();
-  uint32_t *fh = cb.vlist.as<uint32_t>();
-  uint32_t *ph = fh + (n + 1), *fb = ph + (n + 1), *pb = fb + (n + 1);
-  out.vh = pb + (n + 1);
+  out.vh = cb.vlist.as<uint32_t>();
   out.vb = out.vh + (n + 1);
-  uint32_t *fw = out.vb + (n + 1), *pw = fw + (n + 1);
-  out.vw = pw + (n + 1);
-  uint32_t *fwb = out.vw + (n + 1), *pwb = fwb + (n + 1);
-  out.vwb = pwb + (n + 1);
+  out.vw = out.vb + (n + 1);
+  out.vwb = out.vw + (n + 1);
+  uint32_t *tile_cnt = out.vwb + (n + 1);
+  uint32_t *tile_pos = tile_cnt + 4 * (uint64_t)ntiles;
 
   if (marks) STC_TRY(hipEventRecord(marks[0], st));
   hipLaunchKernelGGL(k_fill_words, dim3(8), dim3(256), 0, st, (uint32_t *)ctrl,
@@ -1367,16 +1423,11 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
                      ctrl);
   STC_TRY(hipGetLastError());
   if (n > 0) {
-    const int vgrid = grid_for(n, 256, num_cus * 8);
-    hipLaunchKernelGGL(k_vflag_split, dim3(vgrid), dim3(256), 0, st, vf, g.off, n,
-                       kWaveMaxDeg, fh, fb, fw, fwb);
+    hipLaunchKernelGGL(k_vcount, dim3(ntiles), dim3(256), 0, st, vf, g.off, n, tile_cnt);
     STC_TRY(hipGetLastError());
-    STC_TRY(scan_exclusive_u32(fh, ph, n, cb.scan, st));
-    STC_TRY(scan_exclusive_u32(fb, pb, n, cb.scan, st));
-    STC_TRY(scan_exclusive_u32(fw, pw, n, cb.scan, st));
-    STC_TRY(scan_exclusive_u32(fwb, pwb, n, cb.scan, st));
-    hipLaunchKernelGGL(k_vlist_scatter, dim3(vgrid), dim3(256), 0, st, fh, ph, fb, pb, fw,
-                       pw, fwb, pwb, n, out.vh, out.vb, out.vw, out.vwb, ctrl);
+    STC_TRY(scan_exclusive_u32(tile_cnt, tile_pos, 4 * (uint64_t)ntiles, cb.scan, st));
+    hipLaunchKernelGGL(k_vscatter, dim3(ntiles), dim3(256), 0, st, vf, g.off, n, tile_cnt,
+                       tile_pos, out.vh, out.vb, out.vw, out.vwb, ctrl);
     STC_TRY(hipGetLastError());
   }
   if (marks) STC_TRY(hipEventRecord(marks[1], st));
