@@ -134,9 +134,16 @@ int stc_last_classes(const stc_ctx *ctx, stc_classes *out);
 int stc_prepare(stc_ctx *ctx, const stc_edges *g, int g_is_device);
 int stc_count_prepared(stc_ctx *ctx, const stc_opts *opts, stc_counts *out);
 
-/* Count k independent graphs in one persistent launch; out has k entries. */
+/* Count k independent graphs in one persistent launch; out has k entries.
+ * Equivalent to stc_prepare_batch followed by stc_count_batch_prepared.  The
+ * reference counts one graph per program run; this has no counterpart there. */
 int stc_count_batch(stc_ctx *ctx, const stc_edges *graphs, uint32_t k,
                     const stc_opts *opts, stc_counts *out);
+/* Two-phase batch: upload the k host edge lists and build their union CSR
+ * once; then count it (eager run, hipGraph capture, opts.repeat replays)
+ * with out receiving the k per-graph counts of the prepared batch. */
+int stc_prepare_batch(stc_ctx *ctx, const stc_edges *graphs, uint32_t k);
+int stc_count_batch_prepared(stc_ctx *ctx, const stc_opts *opts, stc_counts *out);
 
 /* Statistics of the last counting call.  The per-kernel edge and byte counts
  * are computed by one extra device pass on the first call after a count. */

@@ -25,20 +25,24 @@ extern "C" {
 
 enum { STC_TABLE_SKIP = 0, STC_TABLE_MERGE = 1, STC_TABLE_HASH = 2, STC_TABLE_BITMAP = 3 };
 #define STC_BUCKETS 256
-#define STC_MLP_FEATURES 6
+#define STC_MLP_FEATURES 7
 
-/* Oriented edges per bucket of the prepared graph (runs the binning pass). */
+/* Oriented edges per bucket of the prepared graph, or of the prepared batch's
+ * union graph (runs the binning pass). */
 int stc_bucket_histogram(stc_ctx *ctx, uint32_t out[STC_BUCKETS]);
 
-/* Count the prepared graph with an explicit strategy per bucket
- * (STC_TABLE_*); buckets set to STC_TABLE_SKIP are not counted, so the result
- * is partial unless every non-empty bucket has a strategy. */
+/* Count the prepared graph (or the prepared batch, through the batched
+ * kernel, with the per-graph counts summed) with an explicit strategy per
+ * bucket (STC_TABLE_*); buckets set to STC_TABLE_SKIP are not counted, so the
+ * result is partial unless every non-empty bucket has a strategy. */
 int stc_count_with_table(stc_ctx *ctx, const uint8_t table[STC_BUCKETS], int repeat,
                          stc_counts *out);
 
-/* Dispatcher features of bucket `bucket` holding `count` edges in a graph of
- * n vertices and m oriented edges (the same vector the device computes). */
-void stc_bucket_features(int bucket, uint32_t count, uint32_t n, uint64_t m,
+/* Dispatcher features of bucket `bucket` holding `count` edges when k graphs
+ * (k = 1: one graph) with n vertices and m oriented edges in total are
+ * counted together: la, lb, la - lb, log2(count + 1), log2(n/k + 1),
+ * log2(m/k + 1), log2 k. */
+void stc_bucket_features(int bucket, uint32_t count, uint32_t n, uint64_t m, uint32_t k,
                          float out[STC_MLP_FEATURES]);
 
 /* Train the dispatcher MLP (6 -> 16 ReLU -> 3) on the device by full-batch

@@ -39,7 +39,8 @@ KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitm
 EXPORTED = [
     "stc_create", "stc_destroy", "stc_last_error", "stc_model_status",
     "stc_count", "stc_count_device", "stc_last_classes", "stc_prepare",
-    "stc_count_prepared", "stc_count_batch", "stc_get_stats", "stc_run",
+    "stc_count_prepared", "stc_count_batch", "stc_prepare_batch",
+    "stc_count_batch_prepared", "stc_get_stats", "stc_run",
     "stc_load_graph", "stc_parse_graph", "stc_write_graph", "stc_graph_free",
     "stc_gen_er", "stc_gen_grid", "stc_gen_rmat", "stc_gen_hub", "stc_gen_batch",
     "stc_batch_free", "stc_gen_workload", "stc_gen_workload_batch",
@@ -131,6 +132,8 @@ def lib():
     L.stc_prepare.argtypes = [C.c_void_p, P(Edges), C.c_int]
     L.stc_count_prepared.argtypes = [C.c_void_p, P(Opts), P(Counts)]
     L.stc_count_batch.argtypes = [C.c_void_p, P(Edges), C.c_uint32, P(Opts), P(Counts)]
+    L.stc_prepare_batch.argtypes = [C.c_void_p, P(Edges), C.c_uint32]
+    L.stc_count_batch_prepared.argtypes = [C.c_void_p, P(Opts), P(Counts)]
     L.stc_last_classes.argtypes = [C.c_void_p, P(Classes)]
     L.stc_get_stats.argtypes = [C.c_void_p, P(Stats)]
     L.stc_run.argtypes = [C.c_void_p, P(Edges), P(C.c_uint64), P(C.c_uint64)]
@@ -155,7 +158,8 @@ def lib():
     L.stc_gen_workload_batch.argtypes = [C.c_uint32, C.c_int, P(P(_HostBatch))]
     L.stc_bucket_histogram.argtypes = [C.c_void_p, C.c_void_p]
     L.stc_count_with_table.argtypes = [C.c_void_p, C.c_void_p, C.c_int, P(Counts)]
-    L.stc_bucket_features.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p]
+    L.stc_bucket_features.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                      C.c_void_p]
     L.stc_bucket_features.restype = None
     L.stc_train_dispatcher.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                        C.c_float, C.c_uint64, C.c_char_p]
@@ -403,7 +407,7 @@ class Counter:
     def train_dispatcher(self, feat, cost, out_path, iters=3000, lr=0.01, seed=1):
         f = np.ascontiguousarray(feat, np.float32)
         c = np.ascontiguousarray(cost, np.float32)
-        assert f.shape[1] == 6 and c.shape[1] == 3 and f.shape[0] == c.shape[0]
+        assert f.shape[1] == 7 and c.shape[1] == 3 and f.shape[0] == c.shape[0]
         _check(lib().stc_train_dispatcher(self._h, f.ctypes.data, c.ctypes.data, f.shape[0],
                                           iters, lr, seed, out_path.encode()), self._h)
 
@@ -415,24 +419,42 @@ class Counter:
     def set_tune(self, text):
         _check(lib().stc_set_tune(self._h, text.encode()), self._h)
 
-    def count_batch(self, batch, dispatch="learned", repeat=1):
-        """Per-graph (balanced, unbalanced) for a Batch or a list of Graphs."""
+    @staticmethod
+    def _batch_edges(batch):
         if isinstance(batch, Batch):
             k = batch.k
-            arr = (Edges * k)(*[batch.graph_edges(g) for g in range(k)])
-        else:
-            k = len(batch)
-            arr = (Edges * k)(*[g.edges() for g in batch])
+            return k, (Edges * k)(*[batch.graph_edges(g) for g in range(k)])
+        k = len(batch)
+        return k, (Edges * k)(*[g.edges() for g in batch])
+
+    def count_batch(self, batch, dispatch="learned", repeat=1):
+        """Per-graph (balanced, unbalanced) for a Batch or a list of Graphs."""
+        k, arr = self._batch_edges(batch)
         out = (Counts * max(k, 1))()
         _check(lib().stc_count_batch(self._h, arr, k,
                                      C.byref(self._opts(dispatch, repeat, False)), out),
                self._h)
         return [(int(out[i].balanced), int(out[i].unbalanced)) for i in range(k)]
 
+    def prepare_batch(self, batch):
+        """Upload a batch and build its union CSR (stc_prepare_batch)."""
+        k, arr = self._batch_edges(batch)
+        _check(lib().stc_prepare_batch(self._h, arr, k), self._h)
+        self._batch_k = k
 
-def bucket_features(bucket, count, n, m):
-    out = np.zeros(6, np.float32)
-    lib().stc_bucket_features(bucket, count, n, m, out.ctypes.data)
+    def count_batch_prepared(self, dispatch="learned", repeat=1):
+        k = self._batch_k
+        out = (Counts * max(k, 1))()
+        _check(lib().stc_count_batch_prepared(self._h, C.byref(self._opts(dispatch, repeat, False)),
+                                              out), self._h)
+        return [(int(out[i].balanced), int(out[i].unbalanced)) for i in range(k)]
+
+
+def bucket_features(bucket, count, n, m, k=1):
+    """Dispatcher features of a bucket when k graphs of n vertices and m
+    oriented edges in total are counted together (k = 1: one graph)."""
+    out = np.zeros(7, np.float32)
+    lib().stc_bucket_features(bucket, count, n, m, k, out.ctypes.data)
     return out
 
 

@@ -35,7 +35,9 @@ struct stc_ctx {
   CountBuffers cb;
   BuildResult br;
   DevCsr g;
-  bool prepared = false;
+  bool prepared = false;        // a single graph's CSR is on the device
+  bool batch_ready = false;     // a batch's union CSR is on the device
+  uint32_t batch_k = 0;         // graphs in that batch
   bool kstats_pending = false;  // per-kernel statistics of the last run not read yet
   stc_stats stats{};
   stc_classes classes{};
@@ -66,6 +68,17 @@ int hip_fail(stc_ctx *c, hipError_t e, const char *where) {
 constexpr uint64_t kMaxVertices = (1ull << 31) - (1ull << 20);
 // Edge indices plus a load group (256) or a workgroup stride stay below 2^32.
 constexpr uint64_t kMaxLines = (1ull << 32) - (1ull << 20);
+
+// Dispatcher graph features of what is counted: k graphs (k = 1 for one
+// graph) with n vertices and m oriented edges in total.  Sizes are per
+// graph, so a batch of small graphs reads as small graphs, and log2 k tells
+// the batched kernel's context apart.
+void graph_features(uint32_t n, uint64_t m, uint32_t k, float out[3]) {
+  const double kk = k ? (double)k : 1.0;
+  out[0] = (float)std::log2((double)n / kk + 1.0);
+  out[1] = (float)std::log2((double)m / kk + 1.0);
+  out[2] = (float)std::log2(kk);
+}
 
 void default_tune(TuneParams &t) {
   for (int i = 0; i < kBucketSide; i++) t.merge_wave_min_log2[i] = 5;
@@ -210,7 +223,8 @@ std::string fmt_arr(const char *name, const float *v, int n) {
 
 bool write_model(const std::string &path, const MlpModel &m, const std::string &note) {
   std::string body = "stc-dispatch-mlp v1\n";
-  body += "dims 6 16 3\n";
+  body += "dims " + std::to_string(kMlpIn) + " " + std::to_string(kMlpHidden) + " " +
+          std::to_string(kMlpOut) + "\n";
   body += fmt_arr("mean", m.mean, kMlpIn);
   body += fmt_arr("inv_std", m.inv_std, kMlpIn);
   body += fmt_arr("w1", &m.w1[0][0], kMlpHidden * kMlpIn);
@@ -311,6 +325,7 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   if (rc) return rc;
   (void)hipSetDevice(c->device);
   c->prepared = false;
+  c->batch_ready = false;
   const uint32_t *du = g->u, *dv = g->v;
   const int8_t *ds = g->sign;
   hipError_t e = hipSuccess;
@@ -351,6 +366,7 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   c->g.col = c->bb.col.as<uint32_t>();
   c->g.src = c->bb.src.as<uint32_t>();
   c->prepared = true;
+  c->batch_ready = false;
   std::memset(&c->stats, 0, sizeof(c->stats));
   c->kstats_pending = false;
   c->stats.n = c->br.n;
@@ -371,8 +387,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   (void)hipSetDevice(c->device);
   c->hp.mode = mode;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
-  c->hp.graph_feat[0] = (float)std::log2((double)c->g.n + 1.0);
-  c->hp.graph_feat[1] = (float)std::log2((double)c->g.m + 1.0);
+  graph_features(c->g.n, c->g.m, 1, c->hp.graph_feat);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   if (e != hipSuccess) return hip_fail(c, e, "params upload");
   std::vector<float> times, bins, kt[kKernels];
@@ -494,16 +509,10 @@ float stc_run(stc_ctx *c, const stc_edges *g, uint64_t *total_balanced,
 // counters -- is captured once in a hipGraph and replayed.  The first
 // (eager) run allocates every buffer, so the captured sequence allocates
 // nothing; each replay must reproduce the eager per-graph counts.
-extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
-                               const stc_opts *opts, stc_counts *out) {
-  if (!c || (!graphs && k) || (!out && k)) return STC_E_ARG;
-  if (k == 0) return STC_OK;
+// prepare_batch validates, uploads, shifts and builds the union CSR.
+static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
+  if (!c || !graphs || k == 0) return STC_E_ARG;
   (void)hipSetDevice(c->device);
-  int dispatch = opts ? opts->dispatch : STC_LEARNED;
-  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
-  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
-    return fail(c, STC_E_ARG, "bad dispatch");
-  if (opts && opts->classes) return fail(c, STC_E_ARG, "batched counting reports two classes");
   std::vector<uint32_t> vbase(k + 1, 0);
   std::vector<uint64_t> ebase(k + 1, 0);
   bool contiguous = true;
@@ -524,6 +533,7 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
   const uint32_t n = vbase[k];
   if (m > kMaxLines) return fail(c, STC_E_TOO_LARGE, "batch has > 2^32 - 2^20 lines");
   c->prepared = false;
+  c->batch_ready = false;
   hipError_t e = hipSuccess;
   const uint64_t mm = m ? m : 1;
   if ((e = c->bb.in_u.ensure(mm * 4)) != hipSuccess || (e = c->bb.in_v.ensure(mm * 4)) != hipSuccess ||
@@ -583,7 +593,8 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
   c->g.off = c->bb.off.as<uint32_t>();
   c->g.col = c->bb.col.as<uint32_t>();
   c->g.src = c->bb.src.as<uint32_t>();
-  c->prepared = true;
+  c->batch_ready = true;
+  c->batch_k = k;
   std::memset(&c->stats, 0, sizeof(c->stats));
   c->kstats_pending = false;
   c->stats.n = c->br.n;
@@ -593,11 +604,70 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
   c->stats.b_alg = c->br.b_alg;
   c->stats.ms_upload = ms_up;
   c->stats.ms_build = ms_build;
-  // counting phase: eager run, then capture and replay
+  return STC_OK;
+}
+
+// Eager batch runs of the prepared union CSR under dispatch `mode` (host
+// table for D_TABLE): per-graph counts land in the batch counter buffer;
+// ms_count is the median of the runs.  Used by the training hooks.
+static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repeat,
+                            stc_counts *total, CountResult *res_out) {
+  const uint32_t k = c->batch_k;
+  c->hp.mode = mode;
+  if (table) std::memcpy(c->hp.host_table, table, kBuckets);
+  graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
+  hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+  unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
+  const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
+  std::vector<float> times;
+  for (int r = 0; r < repeat && e == hipSuccess; r++) {
+    e = hipEventRecord(c->ev[0], c->st);
+    if (e == hipSuccess)
+      e = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    times.push_back(ms);
+  }
+  std::vector<unsigned long long> got(2ull * k);
+  if (e == hipSuccess) e = hipMemcpyAsync(got.data(), d_pg, 16ull * k, hipMemcpyDeviceToHost, c->st);
+  CountResult res;
+  if (e == hipSuccess) e = read_count(c->cb, false, res, c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "batch count");
+  total->balanced = total->unbalanced = 0;
+  for (uint32_t i = 0; i < k; i++) {
+    total->balanced += got[2 * i];
+    total->unbalanced += got[2 * i + 1];
+  }
+  c->stats.ms_count = median(times);
+  c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->kstats_pending = true;
+  if (res_out) *res_out = res;
+  return STC_OK;
+}
+
+extern "C" int stc_prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
+  return prepare_batch(c, graphs, k);
+}
+
+// Counting phase of the prepared batch: an eager run (which allocates every
+// buffer and gives the counts), then binning plus the persistent kernel
+// captured once in a hipGraph and replayed opts.repeat times; every replay
+// must reproduce the eager per-graph counts.
+extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
+  if (!c || !out) return STC_E_ARG;
+  if (!c->batch_ready) return fail(c, STC_E_ARG, "no prepared batch");
+  (void)hipSetDevice(c->device);
+  const uint32_t k = c->batch_k;
+  int dispatch = opts ? opts->dispatch : STC_LEARNED;
+  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+    return fail(c, STC_E_ARG, "bad dispatch");
+  if (opts && opts->classes) return fail(c, STC_E_ARG, "batched counting reports two classes");
   c->hp.mode = dispatch;
-  c->hp.graph_feat[0] = (float)std::log2((double)c->g.n + 1.0);
-  c->hp.graph_feat[1] = (float)std::log2((double)c->g.m + 1.0);
-  e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+  graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
+  hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
   const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
   if (e == hipSuccess)
@@ -642,22 +712,37 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
     out[i].unbalanced = got[2 * i + 1];
   }
   c->stats.ms_count = median(times);
-  c->kstats_pending = true;
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->kstats_pending = true;
   c->stats.dispatch_used = (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
   c->have_classes = false;
-  c->prepared = false;  // the union CSR is not a single graph to recount
   return STC_OK;
 }
 
+extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
+                               const stc_opts *opts, stc_counts *out) {
+  if (!c || (!graphs && k) || (!out && k)) return STC_E_ARG;
+  if (k == 0) return STC_OK;
+  int dispatch = opts ? opts->dispatch : STC_LEARNED;
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+    return fail(c, STC_E_ARG, "bad dispatch");
+  if (opts && opts->classes) return fail(c, STC_E_ARG, "batched counting reports two classes");
+  int rc = prepare_batch(c, graphs, k);
+  if (rc) return rc;
+  return stc_count_batch_prepared(c, opts, out);
+}
+
 // ---- training / tuning hooks (include/stc_train.h) -------------------------
+// Both hooks act on whatever is prepared: one graph (stc_prepare) or a batch
+// (stc_prepare_batch, counted by the batched kernel, counts summed).
 extern "C" int stc_bucket_histogram(stc_ctx *c, uint32_t out[STC_BUCKETS]) {
   if (!c || !out) return STC_E_ARG;
   uint8_t skip[kBuckets];
   std::memset(skip, 0, sizeof skip);
   stc_counts cnt{0, 0};
   CountResult res;
-  int rc = count_core(c, D_TABLE, skip, 1, false, &cnt, &res);
+  int rc = c->batch_ready ? batch_table_core(c, D_TABLE, skip, 1, &cnt, &res)
+                          : count_core(c, D_TABLE, skip, 1, false, &cnt, &res);
   if (rc) return rc;
   for (int k = 0; k < kBuckets; k++) out[k] = res.bucket_cnt[k];
   return STC_OK;
@@ -668,18 +753,18 @@ extern "C" int stc_count_with_table(stc_ctx *c, const uint8_t table[STC_BUCKETS]
   if (!c || !table || !out) return STC_E_ARG;
   for (int k = 0; k < kBuckets; k++)
     if (table[k] > STC_TABLE_BITMAP) return fail(c, STC_E_ARG, "bad table entry");
+  if (c->batch_ready) return batch_table_core(c, D_TABLE, table, repeat > 0 ? repeat : 1, out, nullptr);
   return count_core(c, D_TABLE, table, repeat > 0 ? repeat : 1, false, out, nullptr);
 }
 
 extern "C" void stc_bucket_features(int bucket, uint32_t count, uint32_t n, uint64_t m,
-                                    float out[STC_MLP_FEATURES]) {
+                                    uint32_t k, float out[STC_MLP_FEATURES]) {
   float la = (float)(bucket / kBucketSide), lb = (float)(bucket % kBucketSide);
   out[0] = la;
   out[1] = lb;
   out[2] = la - lb;
   out[3] = std::log2((float)count + 1.0f);
-  out[4] = (float)std::log2((double)n + 1.0);
-  out[5] = (float)std::log2((double)m + 1.0);
+  graph_features(n, m, k, out + 4);
 }
 
 extern "C" int stc_train_dispatcher(stc_ctx *c, const float *feat, const float *cost,

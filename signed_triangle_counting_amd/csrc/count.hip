@@ -102,7 +102,7 @@ __device__ uint8_t mlp_rule(int bk, uint32_t cnt, const DispatchParams &p) {
   const MlpModel &M = p.model;
   float la = (float)(bk / kBucketSide), lb = (float)(bk % kBucketSide);
   float x[kMlpIn] = {la, lb, la - lb, __log2f((float)cnt + 1.0f), p.graph_feat[0],
-                     p.graph_feat[1]};
+                     p.graph_feat[1], p.graph_feat[2]};
   float h[kMlpHidden];
 #pragma unroll
   for (int j = 0; j < kMlpHidden; j++) {
@@ -491,7 +491,8 @@ __global__ __launch_bounds__(256) void k_vscatter(uint32_t *__restrict__ vflag,
 // list fit, source length for wave or workgroup kernels), with its oriented
 // edge count and d+(u) + d+(v) sum.  Slots follow the launch order
 // (merge-lane, merge-wave, wave-hash, workgroup-hash, wave-bitmap,
-// workgroup-bitmap); slot 7 counts the edges that cannot close a triangle.
+// workgroup-bitmap); slot 6 the edges a partial table leaves out, slot 7 the
+// edges that cannot close a triangle.
 __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
                                                     const DispatchParams *P,
                                                     const uint8_t *__restrict__ bk8) {
@@ -513,8 +514,11 @@ __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
       const uint32_t span = table[bk] == S_BITMAP ? span_of(g.col, la, ea) : 0u;
       const uint8_t s = bucket_strategy(table, tune, ea - la, span, (int)bk);
       const bool small = ea - la <= kWaveMaxDeg;
-      slot = s == S_MERGE_LANE ? 0u : s == S_MERGE_WAVE ? 1u : s == S_HASH ? (small ? 2u : 3u)
-                                                                        : (small ? 4u : 5u);
+      slot = s == S_SKIP         ? 6u  // left out by a partial strategy table
+             : s == S_MERGE_LANE ? 0u
+             : s == S_MERGE_WAVE ? 1u
+             : s == S_HASH       ? (small ? 2u : 3u)
+                                 : (small ? 4u : 5u);
       words = (ea - la) + (g.off[b + 1] - g.off[b]);
     }
     atomicAdd(&s_e[slot], 1ull);

@@ -33,7 +33,7 @@ constexpr int kBucketSide = 16;
 constexpr int kBuckets = kBucketSide * kBucketSide;
 
 // Dispatcher inputs/outputs resident in device memory.
-constexpr int kMlpIn = 6;
+constexpr int kMlpIn = 7;  // la, lb, la - lb, log2 count, log2 n, log2 m, log2 graphs
 constexpr int kMlpHidden = 16;
 constexpr int kMlpOut = 3;  // merge, hash, bitmap
 struct MlpModel {
@@ -72,7 +72,8 @@ struct DispatchParams {
   MlpModel model;
   TuneParams tune;
   uint8_t host_table[kBuckets];  // D_TABLE
-  float graph_feat[2];           // log2 n, log2 m of the graph
+  float graph_feat[3];           // log2 (n/k + 1), log2 (m/k + 1), log2 k for k graphs
+                                 // counted together (k = 1: one graph)
 };
 
 // Device view of the oriented CSR (rank space).

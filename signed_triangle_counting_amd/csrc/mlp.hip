@@ -22,7 +22,7 @@ constexpr int kW1 = 0;
 constexpr int kB1 = kW1 + kMlpHidden * kMlpIn;
 constexpr int kW2 = kB1 + kMlpHidden;
 constexpr int kB2 = kW2 + kMlpOut * kMlpHidden;
-constexpr int kParams = kB2 + kMlpOut;  // 163
+constexpr int kParams = kB2 + kMlpOut;  // 179
 constexpr int kLanes = 64;
 constexpr int kOwned = (kParams + kLanes - 1) / kLanes;
 

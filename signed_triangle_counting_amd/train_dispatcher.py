@@ -3,12 +3,13 @@
     python -m signed_triangle_counting_amd.train_dispatcher --out DIR [--tune-dir DIR]
 
 For each graph of a synthetic training set (seeds distinct from the benchmark
-workloads), the counter's own binning pass gives the oriented-edge histogram
-over the 256 buckets.  One coordinate-descent pass over the buckets then
+workloads), and for training batches of small graphs counted together by the
+batched kernel, the counter's own binning pass gives the oriented-edge
+histogram over the 256 buckets.  One coordinate-descent pass over the buckets then
 times the whole counting phase with each bucket under each strategy (merge,
 LDS hash, hub bitmap) while every other bucket keeps its current best choice;
 those three in-context times are the bucket's costs.  The samples (bucket
-features, three costs) train the 6-16-3 MLP with the device training kernel
+features, three costs) train the 7-16-3 MLP with the device training kernel
 (stc_train_dispatcher).  The model can only change which kernel counts an
 edge: the evaluation here checks that every dispatch mode returns the same
 counts.
@@ -53,13 +54,24 @@ def training_graphs(quick=False):
     return g[:4] + g[6:7] + g[11:12] if quick else g
 
 
+def training_batches(quick=False):
+    """(name, generator thunk) pairs of batches counted by the batched kernel:
+    small ER / RMAT graphs as in the benchmark batch, other seeds and sizes."""
+    b = [
+        ("batch_512", lambda: stc.gen_batch(512, seed=121)),
+        ("batch_256_dense", lambda: stc.gen_batch(256, d_lo=16.0, d_hi=64.0, seed=122)),
+        ("batch_1024_small", lambda: stc.gen_batch(1024, n_hi=8192, seed=123)),
+    ]
+    return b[:1] if quick else b
+
+
 def time_table(ctx, table, reps):
     ctx.count_with_table(table, repeat=1)
     ctx.count_with_table(table, repeat=reps)
     return float(ctx.stats()["ms_count"])
 
 
-def collect(ctx, name, g, reps=3, log=print):
+def collect(ctx, name, g, reps=3, log=print, batch=False):
     """In-context costs by one coordinate-descent pass over the buckets.
 
     Start from the best single-strategy table; visit buckets heaviest first
@@ -69,7 +81,12 @@ def collect(ctx, name, g, reps=3, log=print):
     Per-vertex staging is shared by a vertex's buckets, so costs measured in
     context, not bucket by bucket in isolation, are what the choice changes.
     """
-    ctx.prepare(g)
+    if batch:
+        ctx.prepare_batch(g)
+        k = g.k
+    else:
+        ctx.prepare(g)
+        k = 1
     st = ctx.stats()
     n, m = st["n"], st["m_oriented"]
     hist = ctx.bucket_histogram()
@@ -92,7 +109,7 @@ def collect(ctx, name, g, reps=3, log=print):
             t[b] = stc.TABLE[s]
             c.append(time_table(ctx, t, reps))
         table[b] = stc.TABLE[STRATS[int(np.argmin(c))]]
-        feats.append(stc.bucket_features(b, int(hist[b]), n, m))
+        feats.append(stc.bucket_features(b, int(hist[b]), n, m, k))
         costs.append(c)
     final = time_table(ctx, table, reps)
     log("  %-14s n=%d m+=%d buckets=%d start=%s %.3fms -> tuned table %.3fms" % (
@@ -100,15 +117,21 @@ def collect(ctx, name, g, reps=3, log=print):
     return np.array(feats, np.float32), np.array(costs, np.float32), order, final
 
 
-def evaluate(ctx, graphs, log=print):
+def evaluate(ctx, graphs, log=print, batch=False):
     rows = []
     for name, thunk in graphs:
         g = thunk()
-        ctx.prepare(g)
+        if batch:
+            ctx.prepare_batch(g)
+        else:
+            ctx.prepare(g)
         res = {}
         for mode in ("fixed", "learned", "merge", "hash", "bitmap"):
-            ctx.count_prepared(dispatch=mode, repeat=2)
-            cnt = ctx.count_prepared(dispatch=mode, repeat=5)
+            if batch:
+                cnt = tuple(map(tuple, ctx.count_batch_prepared(dispatch=mode, repeat=5)))
+            else:
+                ctx.count_prepared(dispatch=mode, repeat=2)
+                cnt = ctx.count_prepared(dispatch=mode, repeat=5)
             res[mode] = (ctx.stats()["ms_count"], cnt)
         counts = {v[1] for v in res.values()}
         if len(counts) != 1:
@@ -138,9 +161,11 @@ def main():
     feats, costs, meta = [], [], []
     print("collecting per-bucket strategy timings", flush=True)
     graphs = training_graphs(args.quick)
+    batches = training_batches(args.quick)
     tuned = {}
-    for name, thunk in graphs:
-        f, c, b, final = collect(ctx, name, thunk(), log=lambda s: print(s, flush=True))
+    for (name, thunk), is_batch in [(x, False) for x in graphs] + [(x, True) for x in batches]:
+        f, c, b, final = collect(ctx, name, thunk(), log=lambda s: print(s, flush=True),
+                                 batch=is_batch)
         feats.append(f)
         costs.append(c)
         meta += [(name, x) for x in b]
@@ -155,6 +180,7 @@ def main():
     # agreement with the per-sample best strategy (training set)
     ctx.load_models(args.out)
     rows = evaluate(ctx, graphs, log=lambda s: print(s, flush=True))
+    rows += evaluate(ctx, batches, log=lambda s: print(s, flush=True), batch=True)
     for r in rows:
         r["coordinate_descent_table"] = tuned.get(r["graph"])
     with open(os.path.join(args.out, "dispatch_eval.json"), "w") as f:

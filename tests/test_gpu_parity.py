@@ -315,3 +315,27 @@ def test_kernel_statistics_account_for_every_edge(ctx):
                 st["edges_skipped"]) == st["m_oriented"] == o.m
         assert st["edges_skipped"] == int((~counted).sum())
         assert sum(st["bytes_kernel"]) == 4 * words
+
+
+def test_batch_two_phase_and_training_hooks(ctx):
+    """stc_prepare_batch + stc_count_batch_prepared equals stc_count_batch in
+    every mode; the bucket histogram and table-driven counts of a prepared
+    batch cover all its edges and sum to the per-graph totals."""
+    b = stc.gen_batch(64, n_hi=4096, seed=901)
+    want = ctx.count_batch(b, dispatch="merge")
+    ctx.prepare_batch(b)
+    for mode in MODES:
+        say("batch two-phase", mode)
+        assert ctx.count_batch_prepared(dispatch=mode, repeat=2) == want, mode
+    total = (sum(x[0] for x in want), sum(x[1] for x in want))
+    hist = ctx.bucket_histogram()
+    st = ctx.stats()
+    assert int(hist.sum()) + st["edges_skipped"] == st["m_oriented"]
+    nz = np.nonzero(hist)[0]
+    for s in ("merge", "hash", "bitmap"):
+        t = np.zeros(256, np.uint8)
+        t[nz] = stc.TABLE[s]
+        say("batch table", s)
+        assert ctx.count_with_table(t) == total, s
+    # prepared batch stays countable after the hooks
+    assert ctx.count_batch_prepared(dispatch="learned") == want
