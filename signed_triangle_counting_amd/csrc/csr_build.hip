@@ -208,15 +208,33 @@ __global__ void k_compact_degree(const uint64_t *__restrict__ key,
                                  int nb, uint64_t *__restrict__ ukey,
                                  uint32_t *__restrict__ deg) {
   const uint64_t mask = (1ull << nb) - 1ull;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+  const uint32_t lane = __lane_id();
+  const uint64_t mround = (m + 63) & ~63ull;  // keep waves converged
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mround;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    if (!flag[i]) continue;
-    uint64_t k = key[i];
-    ukey[pos[i]] = k;
-    uint32_t lo = (uint32_t)((k >> (nb + 1)) & mask);
-    uint32_t hi = (uint32_t)((k >> 1) & mask);
-    atomicAdd(&deg[lo], 1u);
-    atomicAdd(&deg[hi], 1u);
+    const bool in = i < m;
+    bool keep = false;
+    uint32_t lo = 0xFFFFFFFFu;
+    if (in) {
+      keep = flag[i] != 0;
+      const uint64_t k = key[i];
+      lo = (uint32_t)((k >> (nb + 1)) & mask);
+      if (keep) {
+        ukey[pos[i]] = k;
+        atomicAdd(&deg[(uint32_t)((k >> 1) & mask)], 1u);  // upper endpoints: scattered
+      }
+    }
+    // lower endpoints are sorted: one atomic per run of equal lo in the wave
+    const uint32_t prev = (uint32_t)__shfl_up((int)lo, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || prev != lo);
+    const uint64_t kept = __ballot(keep);
+    if ((heads >> lane) & 1u) {
+      const uint64_t above = heads & ~((2ull << lane) - 1ull);
+      const uint64_t run = (above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull) &
+                           ~((1ull << lane) - 1ull);
+      const uint32_t c = (uint32_t)__popcll(kept & run);
+      if (c) atomicAdd(&deg[lo], c);
+    }
   }
 }
 
@@ -266,8 +284,7 @@ __global__ void k_rank(const uint64_t *__restrict__ vkey, uint32_t n, int bn,
 
 __global__ void k_orient(const uint64_t *__restrict__ ukey, uint64_t m, int nb,
                          const uint32_t *__restrict__ rank,
-                         uint64_t *__restrict__ key2,
-                         uint32_t *__restrict__ fdeg) {
+                         uint64_t *__restrict__ key2) {
   const uint64_t mask = (1ull << nb) - 1ull;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -276,18 +293,37 @@ __global__ void k_orient(const uint64_t *__restrict__ ukey, uint64_t m, int nb,
     uint32_t rb = rank[(k >> 1) & mask];
     uint32_t s = ra < rb ? ra : rb, d = ra < rb ? rb : ra;
     key2[i] = ((uint64_t)s << 32) | ((uint64_t)d << 1) | (k & 1ull);
-    atomicAdd(&fdeg[s], 1u);
   }
 }
 
+// Split the (src, dst)-sorted keys into col / src and count out-degrees: the
+// sources are sorted, so each run of equal sources in a wave adds its length
+// with one atomic.
 __global__ void k_split_csr(const uint64_t *__restrict__ key2, uint64_t m,
                             uint32_t *__restrict__ col,
-                            uint32_t *__restrict__ src) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+                            uint32_t *__restrict__ src,
+                            uint32_t *__restrict__ fdeg) {
+  const uint32_t lane = __lane_id();
+  const uint64_t mround = (m + 63) & ~63ull;  // keep waves converged
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mround;
        i += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t k = key2[i];
-    col[i] = (uint32_t)k;
-    src[i] = (uint32_t)(k >> 32);
+    const bool in = i < m;
+    uint32_t a = 0xFFFFFFFFu;
+    if (in) {
+      const uint64_t k = key2[i];
+      a = (uint32_t)(k >> 32);
+      col[i] = (uint32_t)k;
+      src[i] = a;
+    }
+    const uint32_t prev = (uint32_t)__shfl_up((int)a, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || prev != a);
+    const uint64_t valid = __ballot(in);
+    if (in && ((heads >> lane) & 1u)) {
+      const uint64_t above = heads & ~((2ull << lane) - 1ull);
+      const uint64_t run = (above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull) &
+                           ~((1ull << lane) - 1ull);
+      atomicAdd(&fdeg[a], (uint32_t)__popcll(valid & run));
+    }
   }
 }
 
@@ -537,8 +573,7 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
   uint32_t *fdeg = bb.fdeg.as<uint32_t>();
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1, T)), dim3(T), 0, st, fdeg,
                      (uint64_t)n + 1, 0u);
-  hipLaunchKernelGGL(k_orient, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m, nb, rank, kB,
-                     fdeg);
+  hipLaunchKernelGGL(k_orient, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m, nb, rank, kB);
   STC_TRY(hipGetLastError());
   std::swap(kA, kB);
   std::vector<int> sh = shifts_for(1, nb);
@@ -547,7 +582,7 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
   STC_TRY(bb.col.ensure((m ? m : 1) * 4));
   STC_TRY(bb.src.ensure((m ? m : 1) * 4));
   hipLaunchKernelGGL(k_split_csr, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m,
-                     bb.col.as<uint32_t>(), bb.src.as<uint32_t>());
+                     bb.col.as<uint32_t>(), bb.src.as<uint32_t>(), fdeg);
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(fdeg, bb.off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
   // algorithmic bytes and max out-degree
