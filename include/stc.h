@@ -134,6 +134,15 @@ int stc_last_classes(const stc_ctx *ctx, stc_classes *out);
 int stc_prepare(stc_ctx *ctx, const stc_edges *g, int g_is_device);
 int stc_count_prepared(stc_ctx *ctx, const stc_opts *opts, stc_counts *out);
 
+/* Per-vertex four-class counts of the prepared graph (stc_prepare): cnt has
+ * 4 n entries, and cnt[4 v + k] counts the T_k triangles (k = 0..3: t0..t3 of
+ * stc_classes) whose lowest-ranked vertex in the (degree, id) orientation is
+ * vertex v.  This is the per-vertex array of the reference's four-class
+ * program (tc_serial.cpp:141-158), which that program only sums.  out
+ * (nullable) receives the balanced / unbalanced totals; the classes are
+ * available from stc_last_classes. */
+int stc_count_per_vertex(stc_ctx *ctx, const stc_opts *opts, uint64_t *cnt, stc_counts *out);
+
 /* Count k independent graphs in one persistent launch; out has k entries.
  * Equivalent to stc_prepare_batch followed by stc_count_batch_prepared.  The
  * reference counts one graph per program run; this has no counterpart there. */

@@ -40,7 +40,7 @@ EXPORTED = [
     "stc_create", "stc_destroy", "stc_last_error", "stc_model_status",
     "stc_count", "stc_count_device", "stc_last_classes", "stc_prepare",
     "stc_count_prepared", "stc_count_batch", "stc_prepare_batch",
-    "stc_count_batch_prepared", "stc_get_stats", "stc_run",
+    "stc_count_batch_prepared", "stc_count_per_vertex", "stc_get_stats", "stc_run",
     "stc_load_graph", "stc_parse_graph", "stc_write_graph", "stc_graph_free",
     "stc_gen_er", "stc_gen_grid", "stc_gen_rmat", "stc_gen_hub", "stc_gen_batch",
     "stc_batch_free", "stc_gen_workload", "stc_gen_workload_batch",
@@ -133,6 +133,7 @@ def lib():
     L.stc_count_prepared.argtypes = [C.c_void_p, P(Opts), P(Counts)]
     L.stc_count_batch.argtypes = [C.c_void_p, P(Edges), C.c_uint32, P(Opts), P(Counts)]
     L.stc_prepare_batch.argtypes = [C.c_void_p, P(Edges), C.c_uint32]
+    L.stc_count_per_vertex.argtypes = [C.c_void_p, P(Opts), C.c_void_p, P(Counts)]
     L.stc_count_batch_prepared.argtypes = [C.c_void_p, P(Opts), P(Counts)]
     L.stc_last_classes.argtypes = [C.c_void_p, P(Classes)]
     L.stc_get_stats.argtypes = [C.c_void_p, P(Stats)]
@@ -380,6 +381,16 @@ class Counter:
         _check(lib().stc_count_prepared(self._h, C.byref(self._opts(dispatch, repeat, classes)),
                                         C.byref(out)), self._h)
         return int(out.balanced), int(out.unbalanced)
+
+    def count_per_vertex(self, dispatch="learned", repeat=1):
+        """Per-vertex four-class counts of the prepared graph: an (n, 4) uint64
+        array, row v = T0..T3 triangles whose lowest-ranked vertex is v."""
+        n = self.stats()["n"]
+        cnt = np.zeros((max(n, 1), 4), np.uint64)
+        out = Counts()
+        _check(lib().stc_count_per_vertex(self._h, C.byref(self._opts(dispatch, repeat, True)),
+                                          cnt.ctypes.data, C.byref(out)), self._h)
+        return cnt[:n]
 
     def classes(self):
         c = Classes()

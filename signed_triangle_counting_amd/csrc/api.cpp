@@ -647,6 +647,37 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
   return STC_OK;
 }
 
+extern "C" int stc_count_per_vertex(stc_ctx *c, const stc_opts *opts, uint64_t *cnt,
+                                    stc_counts *out) {
+  if (!c || !cnt) return STC_E_ARG;
+  if (!c->prepared) return fail(c, STC_E_ARG, "no prepared graph");
+  (void)hipSetDevice(c->device);
+  int dispatch = opts ? opts->dispatch : STC_LEARNED;
+  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+    return fail(c, STC_E_ARG, "bad dispatch");
+  const uint32_t n = c->g.n;
+  const uint64_t bytes = 32ull * (n ? n : 1);
+  hipError_t e = c->cb.pv.ensure(bytes);
+  if (e == hipSuccess) e = c->cb.pv_ids.ensure(bytes);
+  if (e != hipSuccess) return hip_fail(c, e, "per-vertex alloc");
+  stc_counts tot{0, 0};
+  c->g.pv = c->cb.pv.as<unsigned long long>();
+  int rc = count_core(c, dispatch, nullptr, repeat, true, &tot, nullptr);
+  c->g.pv = nullptr;
+  if (rc) return rc;
+  if (n) {
+    e = enqueue_pv_to_ids(c->cb.pv.as<unsigned long long>(), c->bb.perm.as<uint32_t>(), n,
+                          c->cb.pv_ids.as<unsigned long long>(), c->num_cus, c->st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(cnt, c->cb.pv_ids.p, 32ull * n, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(c, e, "per-vertex readback");
+  }
+  if (out) *out = tot;
+  return STC_OK;
+}
+
 extern "C" int stc_prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
   return prepare_batch(c, graphs, k);
 }

@@ -587,6 +587,23 @@ struct Acc {
   }
 };
 
+// Per-vertex counts (four-class kernels only): credit source vertex a (rank)
+// with the classes its triangles added to acc since `before` was taken.  acc
+// is wave-uniform; lane 0 adds.
+template <bool CLS>
+__device__ __forceinline__ void pv_credit(const DevCsr &g, uint32_t a, const Acc<CLS> &acc,
+                                          const uint64_t (&before)[4]) {
+  if constexpr (CLS) {
+    if (g.pv && __lane_id() == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint64_t d = acc.c[k] - before[k];
+        if (d) atomicAdd(&g.pv[4ull * a + k], (unsigned long long)d);
+      }
+    }
+  }
+}
+
 // Per-lane probe result folded into wave-uniform edge totals.  q: bit 0 =
 // found, bit 1 = neg(u,x); nb = neg(v,x).
 template <bool CLS>
@@ -650,8 +667,22 @@ __global__ __launch_bounds__(256) void k_merge_lane(DevCsr g, Ctrl *ctrl,
   const uint32_t len = ctrl->len[0];
   uint32_t c[4] = {0, 0, 0, 0};
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride)
-    merge_lane_edge<CLS>(g, list[i], c);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+    const uint32_t e = list[i];
+    if constexpr (CLS) {
+      const uint32_t before[4] = {c[0], c[1], c[2], c[3]};
+      merge_lane_edge<CLS>(g, e, c);
+      if (g.pv) {  // per-vertex counts: credit the edge's source
+        const uint32_t a = g.src[e];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (c[k] != before[k])
+            atomicAdd(&g.pv[4ull * a + k], (unsigned long long)(c[k] - before[k]));
+      }
+    } else {
+      merge_lane_edge<CLS>(g, e, c);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     uint64_t v = c[k];
@@ -725,8 +756,14 @@ __global__ __launch_bounds__(256) void k_merge_wave(DevCsr g, Ctrl *ctrl,
   const uint32_t len = ctrl->len[1];
   const uint32_t nw = gridDim.x * (blockDim.x >> 6);
   Acc<CLS> acc;
-  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < len; i += nw)
-    merge_wave_edge<CLS>(g, __builtin_amdgcn_readfirstlane(list[i]), acc);
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < len; i += nw) {
+    const uint32_t e = __builtin_amdgcn_readfirstlane(list[i]);
+    const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
+    merge_wave_edge<CLS>(g, e, acc);
+    if constexpr (CLS) {
+      if (g.pv) pv_credit<CLS>(g, __builtin_amdgcn_readfirstlane(g.src[e]), acc, before);
+    }
+  }
   acc.flush(ctrl);
 }
 
@@ -933,6 +970,7 @@ __device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, u
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthreads >> 6;
   const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
   const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
   const uint32_t span = span_of(col, la, ea);
   const int lg = table_log2(d, 6);
   const uint32_t cap = 1u << lg, mask = cap - 1u;
@@ -950,6 +988,7 @@ __device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, u
         return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_HASH;
       },
       HashProbe{T, mask}, acc);
+  pv_credit<CLS>(g, a, acc, before);
   __syncthreads();
 }
 
@@ -967,6 +1006,7 @@ __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, ui
   const uint32_t lane = __lane_id();
   const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
   const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
   const uint32_t span = span_of(col, la, ea);
   const int lg = table_log2(d, 4);
   const uint32_t cap = 1u << lg, mask = cap - 1u;
@@ -984,6 +1024,7 @@ __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, ui
         return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == want;
       },
       HashProbe{T, mask}, acc);
+  pv_credit<CLS>(g, a, acc, before);
   wave_lds_sync();
 }
 
@@ -1009,6 +1050,7 @@ __device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a,
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthreads >> 6;
   const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
   const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
   const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
   const uint32_t hi = __builtin_amdgcn_readfirstlane(col[ea - 1] >> 1);
   const uint32_t span = hi - lo + 1u;
@@ -1033,6 +1075,7 @@ __device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a,
         return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_BITMAP;
       },
       BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow}, acc);
+  pv_credit<CLS>(g, a, acc, before);
   __syncthreads();
   for (uint32_t k = tid; k < d; k += nthreads) {
     const uint32_t r = col[la + k] >> 1;
@@ -1053,6 +1096,7 @@ __device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, 
   const uint32_t lane = __lane_id();
   const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
   const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
   const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
   const uint32_t hi = __builtin_amdgcn_readfirstlane(col[ea - 1] >> 1);
   const uint32_t span = hi - lo + 1u;
@@ -1074,6 +1118,7 @@ __device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, 
         return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_BITMAP;
       },
       BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow}, acc);
+  pv_credit<CLS>(g, a, acc, before);
   wave_lds_sync();
   if (in_w) B[o >> 4] = 0u;
   wave_lds_sync();
@@ -1318,6 +1363,16 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
   }
 }
 
+// Per-vertex counts from rank order to vertex-id order (perm: rank -> id).
+__global__ void k_pv_to_ids(const unsigned long long *__restrict__ pv, const uint32_t *__restrict__ perm,
+                            uint32_t n, unsigned long long *__restrict__ out) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const uint64_t x = perm[r];
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[4 * x + k] = pv[4ull * r + k];
+  }
+}
+
 __global__ void k_fill_words(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -1447,6 +1502,11 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
   STC_TRY(enqueue_binning(g, dp, cb, num_cus, st, marks, L));
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
   const int grid = num_cus * 8;
+  if (g.pv) {  // per-vertex counters start from zero on every run
+    hipLaunchKernelGGL(k_fill_words, dim3(grid_for(8ull * g.n, 256, 8192)), dim3(256), 0, st,
+                       (uint32_t *)g.pv, 8ull * g.n, 0u);
+    STC_TRY(hipGetLastError());
+  }
   const size_t hash_lds = hash_lds_bytes(hp.tune);
   const size_t bm_lds = bitmap_lds_bytes(hp.tune);
   const int hash_grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(8, (128 * 1024) / hash_lds));
@@ -1498,6 +1558,13 @@ hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers
                          int num_cus, hipStream_t st) {
   hipLaunchKernelGGL(k_edge_stats, dim3(grid_for(g.m, 256, num_cus * 8)), dim3(256), 0, st, g,
                      cb.gcount.as<Ctrl>(), dp, cb.bk8.as<uint8_t>());
+  return hipGetLastError();
+}
+
+hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *perm, uint32_t n,
+                             unsigned long long *out, int num_cus, hipStream_t st) {
+  hipLaunchKernelGGL(k_pv_to_ids, dim3(grid_for(n, 256, num_cus * 8)), dim3(256), 0, st, pv_rank,
+                     perm, n, out);
   return hipGetLastError();
 }
 

@@ -83,6 +83,10 @@ struct DevCsr {
   const uint32_t *off = nullptr;  // n + 1
   const uint32_t *col = nullptr;  // m: (dst_rank << 1) | neg
   const uint32_t *src = nullptr;  // m: src rank
+  // Per-vertex four-class counters by rank (4 n words: T0..T3), credited to
+  // each triangle's lowest-ranked vertex by the four-class kernels; nullptr
+  // when per-vertex counts are not requested.
+  unsigned long long *pv = nullptr;
 };
 
 // ---- device memory --------------------------------------------------------
@@ -161,7 +165,7 @@ hipError_t shift_batch_ids(uint32_t *u, uint32_t *v, uint64_t m, const uint64_t 
                            hipStream_t st);
 
 struct CountBuffers {
-  DevBuf gcount, lists, blk, vflag, vlist, bk8;
+  DevBuf gcount, lists, blk, vflag, vlist, bk8, pv, pv_ids;
   ScanScratch scan;
 };
 
@@ -190,6 +194,10 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
 // after the run's closing event).
 hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
                          int num_cus, hipStream_t st);
+// Per-vertex counts: pv_rank (4 n, by rank) to out (4 n, by vertex id) through
+// perm (rank -> vertex id).
+hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *perm, uint32_t n,
+                             unsigned long long *out, int num_cus, hipStream_t st);
 // Read back totals and per-kernel sizes after the stream completes.
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r,
                       hipStream_t st);

@@ -339,3 +339,30 @@ def test_batch_two_phase_and_training_hooks(ctx):
         assert ctx.count_with_table(t) == total, s
     # prepared batch stays countable after the hooks
     assert ctx.count_batch_prepared(dispatch="learned") == want
+
+
+@pytest.mark.parametrize("case", ["random", "rmat14", "rmat16", "workload1", "workload2"])
+def test_per_vertex_counts_vs_oracle(ctx, case):
+    """stc_count_per_vertex against the oracle's per-vertex four-class array
+    (the reference's cnt[4 v + k], tc_serial.cpp:141-158), every mode; the
+    rows sum to the four-class totals."""
+    if case == "random":
+        rng = np.random.default_rng(515)
+        n = 3000
+        u, v, s = random_graph(rng, n, 40000, dup=0.1, loops=0.05)
+        g = stc.Graph(u=u, v=v, sign=s, n=n)
+    elif case.startswith("rmat"):
+        g = stc.gen_rmat(int(case[4:]), 16, 0.57, 0.19, 0.19, 0.3, 61)
+    else:
+        g = stc.gen_workload(int(case[-1]))
+    want = oracle.Csr(g.n, g.u, g.v, g.sign).classes_per_vertex()
+    ctx.prepare(g)
+    for mode in MODES:
+        say("per-vertex", case, mode)
+        got = ctx.count_per_vertex(dispatch=mode)
+        assert got.shape == want.shape
+        assert (got == want).all(), mode
+        tot = ctx.classes()
+        assert got.sum(axis=0).tolist() == [tot["t0"], tot["t1"], tot["t2"], tot["t3"]]
+    assert ctx.count_prepared() == (int(want[:, 3].sum() + want[:, 1].sum()),
+                                    int(want[:, 2].sum() + want[:, 0].sum()))

@@ -65,6 +65,30 @@ def brute(n, u, v, s):
     return {"t0": t[0], "t1": t[1], "t2": t[2], "t3": t[3]}
 
 
+def brute_per_vertex(n, u, v, s):
+    """O(n^3) per-vertex classes: each triangle is credited to its vertex of
+    lowest (degree, id) -- the source of its oriented edges (tc_serial.cpp:
+    141-158) -- in column 3 - (number of negative edges)."""
+    sign = {}
+    for a, b, x in zip(u.tolist(), v.tolist(), s.tolist()):
+        if a == b:
+            continue
+        key = (min(a, b), max(a, b))
+        if key not in sign:
+            sign[key] = -1 if x < 0 else 1
+    adj = {}
+    for (a, b), x in sign.items():
+        adj.setdefault(a, {})[b] = x
+        adj.setdefault(b, {})[a] = x
+    per = np.zeros((n, 4), np.int64)
+    for a, b, c in itertools.combinations(sorted(adj), 3):
+        if b in adj[a] and c in adj[a] and c in adj[b]:
+            neg = (adj[a][b] < 0) + (adj[b][c] < 0) + (adj[a][c] < 0)
+            low = min((a, b, c), key=lambda x: (len(adj[x]), x))
+            per[low, 3 - neg] += 1
+    return per
+
+
 def bal_unbal(t):
     return t["t3"] + t["t1"], t["t2"] + t["t0"]
 
@@ -101,6 +125,7 @@ def test_brute_force_tiny(seed):
     assert c.hash() == bal_unbal(want)
     per = c.classes_per_vertex()
     assert per.sum(axis=0).tolist() == [want["t0"], want["t1"], want["t2"], want["t3"]]
+    assert (per.astype(np.int64) == brute_per_vertex(n, u, v, s)).all()
 
 
 # ---- closed forms --------------------------------------------------------------
