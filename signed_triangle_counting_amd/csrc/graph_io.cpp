@@ -9,15 +9,30 @@
 // boundaries as graph_io.h:117-133 does, but each chunk is parsed into its own
 // buffer and the buffers are joined in file order, so a dropped self-loop
 // leaves no hole (graph_io.h:147,159-161 reserve one and never fill it).
+//
+// A buffer that starts with the "%%MatrixMarket" banner is read as a
+// MatrixMarket coordinate file instead, the input of the reference's GPU
+// drivers (GPU Implementations/GlobalMEM/tc_split/signed_tc.cu:21-33, loaded
+// there by Gunrock's io::matrix_market_t, which is not part of the reference):
+// banner "%%MatrixMarket matrix coordinate <real|double|integer|pattern>
+// <general|symmetric>", '%' comment lines, a size line "M N NNZ", then NNZ
+// entries "i j [w]" with 1-based i <= M, j <= N.  An entry's sign is + iff its
+// weight, read as a float, is > 0 (signed_tc.hxx:317); pattern entries are +.
+// n = max(M, N).  Both stored directions of an edge, or one triangle of a
+// symmetric file, give the same undirected edge.  Complex, hermitian,
+// skew-symmetric and array files, indices out of range and an entry count
+// other than NNZ are errors (STC_E_ARG).
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -66,6 +81,155 @@ void parse_range(const char *p, const char *end, Chunk &out) {
   }
 }
 
+// ---- MatrixMarket coordinate files -----------------------------------------
+struct MtxChunk {
+  std::vector<uint32_t> u, v;
+  std::vector<int8_t> s;
+  uint64_t entries = 0;
+  bool bad = false;
+};
+
+const char *next_line(const char *p, const char *end) {
+  while (p < end && *p != '\n') p++;
+  return p < end ? p + 1 : end;
+}
+
+// Lowercase words of one line into out (at most `max` words).
+int line_words(const char *p, const char *end, std::string *out, int max) {
+  int k = 0;
+  while (p < end && *p != '\n' && k < max) {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\r')) p++;
+    if (p >= end || *p == '\n') break;
+    std::string w;
+    while (p < end && *p != ' ' && *p != '\t' && *p != '\r' && *p != '\n')
+      w.push_back((char)std::tolower((unsigned char)*p++));
+    out[k++] = w;
+  }
+  return k;
+}
+
+bool parse_u64(const char *&p, const char *end, uint64_t &x) {
+  while (p < end && (*p == ' ' || *p == '\t')) p++;
+  if (p >= end || *p < '0' || *p > '9') return false;
+  x = 0;
+  while (p < end && *p >= '0' && *p <= '9') {
+    if (x > (~0ull - 9) / 10) return false;
+    x = x * 10 + (uint64_t)(*p++ - '0');
+  }
+  return true;
+}
+
+void parse_mtx_range(const char *p, const char *end, uint64_t rows, uint64_t cols, bool pattern,
+                     MtxChunk &out) {
+  char num[64];
+  while (p < end) {
+    if (*p == '\n' || *p == '\r' || *p == '%') {
+      p = next_line(p, end);
+      continue;
+    }
+    const char *line = p;
+    uint64_t i = 0, j = 0;
+    if (!parse_u64(p, end, i) || !parse_u64(p, end, j) || i < 1 || j < 1 || i > rows ||
+        j > cols) {
+      // a line of only blanks is not an entry
+      const char *q = line;
+      while (q < end && (*q == ' ' || *q == '\t' || *q == '\r')) q++;
+      if (q < end && *q != '\n') out.bad = true;
+      p = next_line(line, end);
+      continue;
+    }
+    int8_t sg = 1;
+    if (!pattern) {
+      while (p < end && (*p == ' ' || *p == '\t')) p++;
+      size_t k = 0;
+      while (p < end && k + 1 < sizeof num && *p != ' ' && *p != '\t' && *p != '\r' &&
+             *p != '\n')
+        num[k++] = *p++;
+      num[k] = 0;
+      char *e = nullptr;
+      const float w = std::strtof(num, &e);
+      if (k == 0 || e == num) {
+        out.bad = true;
+      } else {
+        sg = w > 0.0f ? 1 : -1;
+      }
+    }
+    p = next_line(p, end);
+    out.entries++;
+    if (i == j) continue;
+    out.u.push_back((uint32_t)(i - 1));
+    out.v.push_back((uint32_t)(j - 1));
+    out.s.push_back(sg);
+  }
+}
+
+int parse_mtx(const char *buf, uint64_t len, int threads, stc_host_graph *g) {
+  const char *end = buf + len, *p = buf;
+  std::string w[5];
+  if (line_words(p, end, w, 5) != 5 || w[0] != "%%matrixmarket" || w[1] != "matrix" ||
+      w[2] != "coordinate")
+    return STC_E_ARG;
+  const bool pattern = w[3] == "pattern";
+  if (!pattern && w[3] != "real" && w[3] != "double" && w[3] != "integer") return STC_E_ARG;
+  if (w[4] != "general" && w[4] != "symmetric") return STC_E_ARG;
+  p = next_line(p, end);
+  while (p < end && (*p == '%' || *p == '\n' || *p == '\r')) p = next_line(p, end);
+  uint64_t rows = 0, cols = 0, nnz = 0;
+  if (!parse_u64(p, end, rows) || !parse_u64(p, end, cols) || !parse_u64(p, end, nnz))
+    return STC_E_ARG;
+  if (std::max(rows, cols) > 0xFFFFFFFFull) return STC_E_TOO_LARGE;
+  p = next_line(p, end);
+  const uint64_t rest = (uint64_t)(end - p);
+  int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  if ((uint64_t)nt > rest / 4096 + 1) nt = (int)(rest / 4096 + 1);
+  std::vector<const char *> lo(nt), hi(nt);
+  const uint64_t chunk = rest / nt;
+  for (int t = 0; t < nt; t++) {
+    const char *a = p + (uint64_t)t * chunk;
+    const char *b = (t == nt - 1) ? end : p + (uint64_t)(t + 1) * chunk;
+    if (t > 0)
+      while (a < end && *(a - 1) != '\n') a++;
+    if (t < nt - 1)
+      while (b < end && *(b - 1) != '\n') b++;
+    lo[t] = a;
+    hi[t] = b;
+  }
+  std::vector<MtxChunk> parts(nt);
+  {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t] {
+        if (lo[t] < hi[t]) parse_mtx_range(lo[t], hi[t], rows, cols, pattern, parts[t]);
+      });
+    for (auto &th : pool) th.join();
+  }
+  uint64_t m = 0, entries = 0;
+  for (auto &c : parts) {
+    if (c.bad) return STC_E_ARG;
+    m += c.u.size();
+    entries += c.entries;
+  }
+  if (entries != nnz) return STC_E_ARG;
+  const size_t mm = m ? m : 1;
+  g->u = (uint32_t *)std::malloc(mm * 4);
+  g->v = (uint32_t *)std::malloc(mm * 4);
+  g->sign = (int8_t *)std::malloc(mm);
+  if (!g->u || !g->v || !g->sign) return STC_E_OOM;
+  uint64_t off = 0;
+  for (auto &c : parts) {
+    const size_t k = c.u.size();
+    if (k) {
+      std::memcpy(g->u + off, c.u.data(), k * 4);
+      std::memcpy(g->v + off, c.v.data(), k * 4);
+      std::memcpy(g->sign + off, c.s.data(), k);
+    }
+    off += k;
+  }
+  g->m = m;
+  g->n = (uint32_t)std::max(rows, cols);
+  return STC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -73,6 +237,19 @@ extern "C" {
 int stc_parse_graph(const char *buf, uint64_t len, int threads,
                     stc_host_graph **out) {
   if (!out || (!buf && len)) return STC_E_ARG;
+  static const char kBanner[] = "%%MatrixMarket";
+  if (len >= sizeof kBanner - 1 && std::memcmp(buf, kBanner, sizeof kBanner - 1) == 0) {
+    *out = nullptr;
+    auto *g = (stc_host_graph *)std::calloc(1, sizeof(stc_host_graph));
+    if (!g) return STC_E_OOM;
+    const int rc = parse_mtx(buf, len, threads, g);
+    if (rc != STC_OK) {
+      stc_graph_free(g);
+      return rc;
+    }
+    *out = g;
+    return STC_OK;
+  }
   *out = nullptr;
   auto *g = (stc_host_graph *)std::calloc(1, sizeof(stc_host_graph));
   if (!g) return STC_E_OOM;

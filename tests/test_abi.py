@@ -129,3 +129,68 @@ def test_generator_parameters():
         lo, hi = b.edge_off[gi], b.edge_off[gi + 1]
         assert np.all(b.u[lo:hi] < b.n[gi]) and np.all(b.v[lo:hi] < b.n[gi])
         assert not np.any(b.u[lo:hi] == b.v[lo:hi])
+
+
+# ---- MatrixMarket input (the reference GPU drivers' format) ------------------
+# Hand-written files with the edge lists the MatrixMarket coordinate format
+# fixes: 1-based indices, one size line, NNZ entries; sign + iff the weight,
+# read as a float, is > 0 (GlobalMEM/signed_tc.hxx:317), pattern entries +.
+MTX_CASES = [
+    ("%%MatrixMarket matrix coordinate real general\n% a comment\n4 4 7\n"
+     "1 2 1.0\n2 1 1.0\n2 3 -2.5\n3 2 -2.5\n1 3 0\n3 1 0\n1 4 1e-50\n",
+     4, [(0, 1, 1), (1, 0, 1), (1, 2, -1), (2, 1, -1), (0, 2, -1), (2, 0, -1), (0, 3, -1)]),
+    ("%%MatrixMarket matrix coordinate pattern symmetric\n3 3 3\n2 1\n3 1\n3 2\n",
+     3, [(1, 0, 1), (2, 0, 1), (2, 1, 1)]),
+    ("%%MatrixMarket MATRIX Coordinate Integer General\r\n5 3 2\r\n1 1 4\r\n5 2 -7\r\n",
+     5, [(4, 1, -1)]),                                   # self-loop entry dropped; n = max(M, N)
+    ("%%MatrixMarket matrix coordinate real general\n%\n\n  2 2 1\n  2   1   +3.5  \n",
+     2, [(1, 0, 1)]),
+    ("%%MatrixMarket matrix coordinate real symmetric\n6 6 0\n", 6, []),
+]
+
+
+@pytest.mark.parametrize("text,n,edges", MTX_CASES)
+def test_matrix_market_fixtures(text, n, edges):
+    g = stc.parse_graph(text)
+    assert g.n == n
+    assert list(zip(g.u.tolist(), g.v.tolist(), g.sign.tolist())) == edges
+
+
+@pytest.mark.parametrize("text", [
+    "%%MatrixMarket matrix coordinate real general\n3 3 1\n4 1 1\n",        # row > M
+    "%%MatrixMarket matrix coordinate real general\n3 3 2\n1 2 1\n",        # fewer than NNZ
+    "%%MatrixMarket matrix coordinate real general\n3 3 1\n1 2 1\n2 3 1\n",  # more than NNZ
+    "%%MatrixMarket matrix coordinate real general\n3 3 1\n1 2\n",          # missing weight
+    "%%MatrixMarket matrix coordinate real skew-symmetric\n3 3 1\n2 1 1\n",
+    "%%MatrixMarket matrix coordinate complex general\n3 3 1\n2 1 1 0\n",
+    "%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n",
+    "%%MatrixMarket matrix coordinate real general\n3 3\n",                 # short size line
+])
+def test_matrix_market_rejects(text):
+    with pytest.raises(stc.StcError) as ei:
+        stc.parse_graph(text)
+    assert ei.value.code == stc.STC_E_ARG
+
+
+def test_matrix_market_threaded_and_file(tmp_path):
+    """A large file parses the same with 1 and 8 threads, from memory and from
+    disk, and equals the text-format edge list it was written from."""
+    rng = np.random.default_rng(3)
+    n, m = 5000, 60000
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    w = rng.normal(size=m)
+    lines = ["%%MatrixMarket matrix coordinate real general", "% generated", "%d %d %d" % (n, n, m)]
+    lines += ["%d %d %.6g" % (a + 1, b + 1, x) for a, b, x in zip(u, v, w)]
+    text = "\n".join(lines) + "\n"
+    keep = u != v
+    sign = np.where(np.array([float("%.6g" % x) for x in w]) > 0, 1, -1)
+    want = (u[keep].tolist(), v[keep].tolist(), sign[keep].tolist())
+    for threads in (1, 8):
+        g = stc.parse_graph(text, threads=threads)
+        assert g.n == n
+        assert (g.u.tolist(), g.v.tolist(), g.sign.tolist()) == want
+    p = tmp_path / "g.mtx"
+    p.write_text(text)
+    g = stc.load_graph(str(p))
+    assert (g.u.tolist(), g.v.tolist(), g.sign.tolist()) == want

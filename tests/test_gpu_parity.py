@@ -366,3 +366,28 @@ def test_per_vertex_counts_vs_oracle(ctx, case):
         assert got.sum(axis=0).tolist() == [tot["t0"], tot["t1"], tot["t2"], tot["t3"]]
     assert ctx.count_prepared() == (int(want[:, 3].sum() + want[:, 1].sum()),
                                     int(want[:, 2].sum() + want[:, 0].sum()))
+
+
+def test_matrix_market_input_counts(ctx, tmp_path):
+    """A graph stored as the GPU drivers store it -- MatrixMarket, 1-based,
+    both directions, sign = weight > 0 -- counts the same as its text-format
+    edge list and as the oracle; the CLI reads the .mtx file directly."""
+    g = stc.gen_rmat(13, 16, 0.57, 0.19, 0.19, 0.3, 808)
+    keep = g.u != g.v
+    u, v, s = g.u[keep], g.v[keep], g.sign[keep]
+    want = oracle_counts(g.n, g.u, g.v, g.sign)
+    w = np.where(s < 0, -1.5, 2.0)
+    lines = ["%%MatrixMarket matrix coordinate real general", "%d %d %d" % (g.n, g.n, 2 * len(u))]
+    lines += ["%d %d %g" % (a + 1, b + 1, x) for a, b, x in zip(u, v, w)]
+    lines += ["%d %d %g" % (b + 1, a + 1, x) for a, b, x in zip(u, v, w)]
+    p = tmp_path / "g.mtx"
+    p.write_text("\n".join(lines) + "\n")
+    m = stc.load_graph(str(p))
+    for mode in MODES:
+        say("mtx", mode)
+        assert ctx.count(m, dispatch=mode) == want, mode
+    exe = os.path.join(os.path.dirname(stc.LIB_PATH), "stc")
+    out = subprocess.run([exe, str(p)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "Balanced   = %d" % want[0] in out.stdout
+    assert "Unbalanced = %d" % want[1] in out.stdout
