@@ -996,12 +996,10 @@ __device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, u
 // kWaveCap slots and counts a's hash edges.
 constexpr uint32_t kWaveCap = 4 * kWaveMaxDeg;
 
-// `want` names the strategy whose edges this call counts: S_HASH, or S_BITMAP
-// where a caller has no per-wave bitmap (the batch kernel).
 template <bool CLS>
 __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, uint32_t *T,
                                                  const uint8_t *table, const TuneParams &tune,
-                                                 Acc<CLS> &acc, uint8_t want = S_HASH) {
+                                                 Acc<CLS> &acc) {
   const uint32_t *__restrict__ col = g.col;
   const uint32_t lane = __lane_id();
   const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
@@ -1021,7 +1019,7 @@ __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, ui
   stream_edges<CLS>(
       g, la, ea, 1u,
       [&](uint32_t j, uint32_t pb, uint32_t eb) {
-        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == want;
+        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_HASH;
       },
       HashProbe{T, mask}, acc);
   pv_credit<CLS>(g, a, acc, before);
@@ -1260,12 +1258,16 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 // Items, in queue order (popped from the top, so heaviest kinds first):
 //   merge-lane chunks of kLaneItem edges (one per thread), merge-wave chunks
 //   of one edge per wave, wave-hash and wave-bitmap chunks of one vertex per
-//   wave (both counted through the wave's hash table: a per-wave bitmap
-//   window would cost the workgroup bitmap its LDS), hash vertices, bitmap
-//   vertices.  Every item's triangles are added to the
-//   counters of the graph owning its source vertex (graph_of is indexed by
+//   wave, hash vertices, bitmap vertices.  Every item's triangles are added to
+//   the counters of the graph owning its source vertex (graph_of is indexed by
 //   rank; each graph's ranks are one contiguous range).
-constexpr uint32_t kBatchBlock = 256;
+//   A workgroup works on one item at a time, so all LDS structures share one
+//   region R, kept zero between items: the workgroup bitmap R[0, bitmap_words)
+//   with its low array after it, the workgroup hash table R[0, hash_slots),
+//   the wave hash tables R[w kWaveCap, +kWaveCap) and the wave windows
+//   R[w (window_words + kWaveMaxDeg), ...) with their low arrays.  Bitmaps
+//   clear what they set; tables and low arrays are cleared after use.
+constexpr uint32_t kBatchBlock = 512;
 constexpr uint32_t kLaneItem = kBatchBlock;
 constexpr uint32_t kWavesPerBatchBlock = kBatchBlock / 64;
 
@@ -1293,20 +1295,21 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vh,
     const uint32_t *__restrict__ vb, const uint32_t *__restrict__ vw,
     const uint32_t *__restrict__ vwb, const uint32_t *__restrict__ graph_of,
-    unsigned long long *__restrict__ per_graph, uint32_t bitmap_words, uint32_t hash_slots) {
-  extern __shared__ uint32_t lds[];
-  uint32_t *B = lds;                                  // bitmap, kept zero between items
-  uint32_t *L = B + bitmap_words;                     // low array (kLowCap)
-  uint32_t *T = L + kLowCap;                          // hash table (hash_slots)
-  uint32_t *TW = T + hash_slots;                      // per-wave tables
+    unsigned long long *__restrict__ per_graph, uint32_t bitmap_words, uint32_t hash_slots,
+    uint32_t window_words, uint32_t region_words) {
+  extern __shared__ uint32_t R[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
   const uint32_t tid = threadIdx.x, lane = __lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  table[tid] = ctrl->table[tid];
+  if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
-  for (uint32_t k = tid; k < bitmap_words; k += blockDim.x) B[k] = 0u;
+  for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
+  uint32_t *const TW = R + wave * kWaveCap;                       // wave hash table
+  uint32_t *const BW = R + wave * (window_words + kWaveMaxDeg);   // wave window
+  uint32_t *const LW = BW + window_words;                         // its low array
+  const uint32_t window = window_words * 16u;
   const uint32_t nl = ctrl->len[0], nwv = ctrl->len[1], nh = ctrl->len[2], nb = ctrl->len[3],
                  nhw = ctrl->len[4], nbw = ctrl->len[5];
   const uint32_t i_lane = (nl + kLaneItem - 1) / kLaneItem;
@@ -1341,24 +1344,29 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
       const uint32_t i = (it - b2) * kWavesPerBatchBlock + wave;
       if (i < nhw) {
         const uint32_t a = __builtin_amdgcn_readfirstlane(vw[i]);
-        hash_vertex_wave<false>(g, a, TW + wave * kWaveCap, table, tune, acc, S_HASH);
+        hash_vertex_wave<false>(g, a, TW, table, tune, acc);
         acc.flush_graph(per_graph, graph_of[a]);
+#pragma unroll
+        for (uint32_t k = lane; k < kWaveCap; k += 64) TW[k] = 0u;
       }
-    } else if (it < b3w) {  // wave-bitmap chunk, counted by hash probes
+    } else if (it < b3w) {  // wave-bitmap chunk
       const uint32_t i = (it - b3) * kWavesPerBatchBlock + wave;
       if (i < nbw) {
         const uint32_t a = __builtin_amdgcn_readfirstlane(vwb[i]);
-        hash_vertex_wave<false>(g, a, TW + wave * kWaveCap, table, tune, acc, S_BITMAP);
+        bitmap_vertex_wave<false>(g, a, BW, LW, window, table, tune, acc);
         acc.flush_graph(per_graph, graph_of[a]);
+        LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
       }
     } else if (it < b4) {  // hash vertex
       const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b3w]);
-      hash_vertex_block<false>(g, a, T, table, tune, acc);
+      hash_vertex_block<false>(g, a, R, table, tune, acc);
       acc.flush_graph(per_graph, graph_of[a]);
+      for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
     } else {  // bitmap vertex
       const uint32_t a = __builtin_amdgcn_readfirstlane(vb[it - b4]);
-      bitmap_vertex_block<false>(g, a, B, L, &s_nlow, table, tune, acc);
+      bitmap_vertex_block<false>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
       acc.flush_graph(per_graph, graph_of[a]);
+      for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
     }
   }
 }
@@ -1568,10 +1576,16 @@ hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *
   return hipGetLastError();
 }
 
-size_t batch_lds_bytes(const TuneParams &t) {
-  return 4u * ((size_t)((1u << t.bitmap_span_log2) / 16u) + kLowCap +
-               ((size_t)4u << t.hash_max_log2) + kWavesPerBatchBlock * kWaveCap);
+// Words of the batch kernel's shared LDS region (the largest structure).
+static size_t batch_region_words(const TuneParams &t) {
+  const size_t bitmap = (size_t)((1u << t.bitmap_span_log2) / 16u) + kLowCap;
+  const size_t hash = (size_t)4u << t.hash_max_log2;
+  const size_t waves = (size_t)kWavesPerBatchBlock *
+                       std::max<size_t>(kWaveCap, (1u << t.wave_window_log2) / 16u + kWaveMaxDeg);
+  return std::max(bitmap, std::max(hash, waves));
 }
+
+size_t batch_lds_bytes(const TuneParams &t) { return 4u * batch_region_words(t); }
 
 hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                          CountBuffers &cb, const uint32_t *graph_of,
@@ -1587,7 +1601,8 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
   hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
                      cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vb, L.vw,
                      L.vwb, graph_of, per_graph, (1u << hp.tune.bitmap_span_log2) / 16u,
-                     4u << hp.tune.hash_max_log2);
+                     4u << hp.tune.hash_max_log2, (1u << hp.tune.wave_window_log2) / 16u,
+                     (uint32_t)batch_region_words(hp.tune));
   return hipGetLastError();
 }
 
