@@ -30,7 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitmap"]
+KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitmap",
+           "pull_wave", "pull"]
 WORKLOADS = {1: "er_10k_100k", 2: "grid_960x960_diag", 3: "rmat22_ef16", 4: "hub_18m"}
 
 
@@ -124,8 +125,9 @@ def traffic_from(pmc_dir, workload, out_path):
     fw = copy_factor("copy_write", "WRITE_SIZE")
     fetch, write = read("fetch"), read("write")
     names = {"k_merge_lane": "merge_lane", "k_merge_wave": "merge_wave",
-             "k_hash_wave": "hash_wave", "k_hash<": "hash", "k_bitmap_wave": "bitmap_wave",
-             "k_bitmap<": "bitmap"}
+             "k_hash_wave": "hash_wave", "k_hash<": "hash",
+             "k_bitmap_wave<false, false>": "bitmap_wave", "k_bitmap<false, false>": "bitmap",
+             "k_bitmap_wave<false, true>": "pull_wave", "k_bitmap<false, true>": "pull"}
     out = {}
     try:
         with open(out_path) as f:
@@ -239,7 +241,8 @@ def main():
         "b_alg_bytes": st["b_alg"],
         "roofline": roofline,
         "edges_by_strategy": {"merge": st["edges_merge"], "hash": st["edges_hash"],
-                              "bitmap": st["edges_bitmap"], "skipped": st["edges_skipped"]},
+                              "bitmap": st["edges_bitmap"], "pull": st["edges_pull"],
+                              "skipped": st["edges_skipped"]},
         "model_loaded": ctx.model_status == 0,
     }
     if not args.no_hub:

@@ -72,8 +72,12 @@ typedef enum stc_dispatch {
   STC_FIXED = 1,        /* degree-threshold rule */
   STC_FORCE_MERGE = 2,  /* every edge through the merge kernels */
   STC_FORCE_HASH = 3,   /* every edge through the LDS hash kernel */
-  STC_FORCE_BITMAP = 4  /* every edge through the hub bitmap kernel (edges
+  STC_FORCE_BITMAP = 4, /* every edge through the hub bitmap kernel (edges
                            whose source list does not fit fall back to hash) */
+  STC_FORCE_PULL = 5    /* every edge through the pull kernels (target's list
+                           staged, the source's remaining list streamed; a
+                           target list of more than 1025 entries, or a batch,
+                           falls back to bitmap) */
 } stc_dispatch;
 
 typedef struct stc_opts {
@@ -99,13 +103,14 @@ typedef struct stc_stats {
   uint64_t edges_merge;    /* oriented edges per strategy in the last run */
   uint64_t edges_hash;
   uint64_t edges_bitmap;
+  uint64_t edges_pull;
   uint64_t edges_skipped;  /* edges that cannot close a triangle */
   int dispatch_used;       /* stc_dispatch actually applied */
-  float ms_kernel[6];      /* merge-lane, merge-wave, wave-hash, workgroup-hash,
-                              wave-bitmap and workgroup-bitmap kernel time
-                              (hipEvent on the library stream), median over
-                              repeats */
-  double bytes_kernel[6];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
+  float ms_kernel[8];      /* merge-lane, merge-wave, wave-hash, workgroup-hash,
+                              wave-bitmap, workgroup-bitmap, wave-pull and
+                              workgroup-pull kernel time (hipEvent on the
+                              library stream), median over repeats */
+  double bytes_kernel[8];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
                               oriented edges each of those kernels counted */
 } stc_stats;
 

@@ -23,7 +23,13 @@
 extern "C" {
 #endif
 
-enum { STC_TABLE_SKIP = 0, STC_TABLE_MERGE = 1, STC_TABLE_HASH = 2, STC_TABLE_BITMAP = 3 };
+enum {
+  STC_TABLE_SKIP = 0,
+  STC_TABLE_MERGE = 1,
+  STC_TABLE_HASH = 2,
+  STC_TABLE_BITMAP = 3,
+  STC_TABLE_PULL = 4
+};
 #define STC_BUCKETS 256
 #define STC_MLP_FEATURES 7
 

@@ -30,9 +30,10 @@ STC_E_MODEL = -5
 STC_E_IO = -6
 STC_E_NODEV = -7
 
-DISPATCH = {"learned": 0, "fixed": 1, "merge": 2, "hash": 3, "bitmap": 4}
+DISPATCH = {"learned": 0, "fixed": 1, "merge": 2, "hash": 3, "bitmap": 4, "pull": 5}
 
-KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitmap"]
+KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitmap",
+           "pull_wave", "pull"]
 
 # Names every build of libstc.so must export (the declarations of
 # include/stc.h and include/stc_gen.h).
@@ -48,7 +49,7 @@ EXPORTED = [
     "stc_train_dispatcher", "stc_load_models", "stc_set_tune",
 ]
 
-TABLE = {"skip": 0, "merge": 1, "hash": 2, "bitmap": 3}
+TABLE = {"skip": 0, "merge": 1, "hash": 2, "bitmap": 3, "pull": 4}
 
 
 class StcError(RuntimeError):
@@ -96,9 +97,9 @@ class Stats(C.Structure):
                 ("ms_upload", C.c_float), ("ms_build", C.c_float),
                 ("ms_count", C.c_float), ("ms_count_min", C.c_float),
                 ("ms_bin", C.c_float), ("edges_merge", C.c_uint64),
-                ("edges_hash", C.c_uint64), ("edges_bitmap", C.c_uint64),
+                ("edges_hash", C.c_uint64), ("edges_bitmap", C.c_uint64), ("edges_pull", C.c_uint64),
                 ("edges_skipped", C.c_uint64), ("dispatch_used", C.c_int),
-                ("ms_kernel", C.c_float * 6), ("bytes_kernel", C.c_double * 6)]
+                ("ms_kernel", C.c_float * 8), ("bytes_kernel", C.c_double * 8)]
 
     def as_dict(self):
         d = {}
@@ -418,7 +419,7 @@ class Counter:
     def train_dispatcher(self, feat, cost, out_path, iters=3000, lr=0.01, seed=1):
         f = np.ascontiguousarray(feat, np.float32)
         c = np.ascontiguousarray(cost, np.float32)
-        assert f.shape[1] == 7 and c.shape[1] == 3 and f.shape[0] == c.shape[0]
+        assert f.shape[1] == 7 and c.shape[1] == 4 and f.shape[0] == c.shape[0]
         _check(lib().stc_train_dispatcher(self._h, f.ctypes.data, c.ctypes.data, f.shape[0],
                                           iters, lr, seed, out_path.encode()), self._h)
 

@@ -365,6 +365,8 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   c->g.off = c->bb.off.as<uint32_t>();
   c->g.col = c->bb.col.as<uint32_t>();
   c->g.src = c->bb.src.as<uint32_t>();
+  c->g.tin_off = c->bb.tin_off.as<uint32_t>();
+  c->g.tin = c->bb.tin.as<InEdge>();
   c->prepared = true;
   c->batch_ready = false;
   std::memset(&c->stats, 0, sizeof(c->stats));
@@ -386,6 +388,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   if (!c->prepared) return fail(c, STC_E_ARG, "no prepared graph");
   (void)hipSetDevice(c->device);
   c->hp.mode = mode;
+  c->hp.tune.allow_pull = 1;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   graph_features(c->g.n, c->g.m, 1, c->hp.graph_feat);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
@@ -439,7 +442,7 @@ int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
   int dispatch = opts ? opts->dispatch : STC_LEARNED;
   int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
   bool classes = opts && opts->classes;
-  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_PULL)
     return fail(c, STC_E_ARG, "bad dispatch");
   return count_core(c, dispatch, nullptr, repeat, classes, out, nullptr);
 }
@@ -476,11 +479,13 @@ int stc_get_stats(stc_ctx *c, stc_stats *out) {
     if (e == hipSuccess) e = read_count(c->cb, false, res, c->st);
     if (e != hipSuccess) return hip_fail(c, e, "count statistics");
     // kernel order: merge-lane, merge-wave, wave-hash, workgroup-hash,
-    // wave-bitmap, workgroup-bitmap; slot 7 counts skipped edges
-    c->stats.edges_skipped = res.kern_edges[7];
+    // wave-bitmap, workgroup-bitmap, wave-pull, workgroup-pull; slot 9 counts
+    // the edges that cannot close a triangle
+    c->stats.edges_skipped = res.kern_edges[9];
     c->stats.edges_merge = res.kern_edges[0] + res.kern_edges[1];
     c->stats.edges_hash = res.kern_edges[2] + res.kern_edges[3];
     c->stats.edges_bitmap = res.kern_edges[4] + res.kern_edges[5];
+    c->stats.edges_pull = res.kern_edges[6] + res.kern_edges[7];
     for (int k = 0; k < kKernels; k++) c->stats.bytes_kernel[k] = 4.0 * (double)res.kern_words[k];
     c->kstats_pending = false;
   }
@@ -593,6 +598,8 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
   c->g.off = c->bb.off.as<uint32_t>();
   c->g.col = c->bb.col.as<uint32_t>();
   c->g.src = c->bb.src.as<uint32_t>();
+  c->g.tin_off = c->bb.tin_off.as<uint32_t>();
+  c->g.tin = c->bb.tin.as<InEdge>();
   c->batch_ready = true;
   c->batch_k = k;
   std::memset(&c->stats, 0, sizeof(c->stats));
@@ -614,6 +621,7 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
                             stc_counts *total, CountResult *res_out) {
   const uint32_t k = c->batch_k;
   c->hp.mode = mode;
+  c->hp.tune.allow_pull = 0;  // the batched kernel has no pull items
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
@@ -654,7 +662,7 @@ extern "C" int stc_count_per_vertex(stc_ctx *c, const stc_opts *opts, uint64_t *
   (void)hipSetDevice(c->device);
   int dispatch = opts ? opts->dispatch : STC_LEARNED;
   int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
-  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_PULL)
     return fail(c, STC_E_ARG, "bad dispatch");
   const uint32_t n = c->g.n;
   const uint64_t bytes = 32ull * (n ? n : 1);
@@ -693,10 +701,11 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
   const uint32_t k = c->batch_k;
   int dispatch = opts ? opts->dispatch : STC_LEARNED;
   int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
-  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_PULL)
     return fail(c, STC_E_ARG, "bad dispatch");
   if (opts && opts->classes) return fail(c, STC_E_ARG, "batched counting reports two classes");
   c->hp.mode = dispatch;
+  c->hp.tune.allow_pull = 0;  // the batched kernel has no pull items
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
@@ -755,7 +764,7 @@ extern "C" int stc_count_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k,
   if (!c || (!graphs && k) || (!out && k)) return STC_E_ARG;
   if (k == 0) return STC_OK;
   int dispatch = opts ? opts->dispatch : STC_LEARNED;
-  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_BITMAP)
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_PULL)
     return fail(c, STC_E_ARG, "bad dispatch");
   if (opts && opts->classes) return fail(c, STC_E_ARG, "batched counting reports two classes");
   int rc = prepare_batch(c, graphs, k);
@@ -783,7 +792,7 @@ extern "C" int stc_count_with_table(stc_ctx *c, const uint8_t table[STC_BUCKETS]
                                     stc_counts *out) {
   if (!c || !table || !out) return STC_E_ARG;
   for (int k = 0; k < kBuckets; k++)
-    if (table[k] > STC_TABLE_BITMAP) return fail(c, STC_E_ARG, "bad table entry");
+    if (table[k] > STC_TABLE_PULL) return fail(c, STC_E_ARG, "bad table entry");
   if (c->batch_ready) return batch_table_core(c, D_TABLE, table, repeat > 0 ? repeat : 1, out, nullptr);
   return count_core(c, D_TABLE, table, repeat > 0 ? repeat : 1, false, out, nullptr);
 }

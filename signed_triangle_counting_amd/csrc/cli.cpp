@@ -4,7 +4,7 @@
 // (timer.h:24-26) and the same three result lines
 //   Balanced   = %llu / Unbalanced = %llu / Total      = %llu
 // (tc_merge_sign_enu.cpp:146-149).  Usage errors and unreadable files return
-// 1.  Optional flags: --dispatch=learned|fixed|merge|hash|bitmap, --repeat N,
+// 1.  Optional flags: --dispatch=learned|fixed|merge|hash|bitmap|pull, --repeat N,
 // --classes (adds the T0..T3 block of tc_serial.cpp:18-30), --json,
 // --device N, --model-dir DIR.
 #include <chrono>
@@ -38,6 +38,7 @@ int main(int argc, char **argv) {
       else if (d == "merge") dispatch = STC_FORCE_MERGE;
       else if (d == "hash") dispatch = STC_FORCE_HASH;
       else if (d == "bitmap") dispatch = STC_FORCE_BITMAP;
+      else if (d == "pull") dispatch = STC_FORCE_PULL;
       else {
         std::fprintf(stderr, "unknown dispatch %s\n", d.c_str());
         return 1;
@@ -120,12 +121,12 @@ int main(int argc, char **argv) {
         "{\"n\": %u, \"m_in\": %llu, \"m_oriented\": %llu, \"balanced\": %llu, "
         "\"unbalanced\": %llu, \"ms_upload\": %.4f, \"ms_build\": %.4f, \"ms_count\": %.4f, "
         "\"b_alg\": %.0f, \"edges_merge\": %llu, \"edges_hash\": %llu, \"edges_bitmap\": %llu, "
-        "\"dispatch\": %d}\n",
+        "\"edges_pull\": %llu, \"dispatch\": %d}\n",
         st.n, (unsigned long long)st.m_in, (unsigned long long)st.m_oriented,
         (unsigned long long)cnt.balanced, (unsigned long long)cnt.unbalanced, st.ms_upload,
         st.ms_build, st.ms_count, st.b_alg, (unsigned long long)st.edges_merge,
         (unsigned long long)st.edges_hash, (unsigned long long)st.edges_bitmap,
-        st.dispatch_used);
+        (unsigned long long)st.edges_pull, st.dispatch_used);
   }
   stc_destroy(ctx);
   stc_graph_free(g);

@@ -42,21 +42,21 @@ constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kerne
 struct Ctrl {
   uint32_t bucket_cnt[kBuckets];
   uint32_t done;
-  uint32_t len[6];   // merge-lane list, merge-wave list, hash vertices (workgroup),
-                     // bitmap vertices (workgroup), hash vertices (wave), bitmap
-                     // vertices (wave)
+  uint32_t len[8];   // merge-lane list, merge-wave list, then vertex lists:
+                     // workgroup hash, workgroup bitmap, wave hash, wave bitmap,
+                     // workgroup pull, wave pull
   uint32_t work[8];  // dynamic work counters
   uint32_t pad[1];
-  uint32_t queue[5][8][32];          // XCD work queues (workgroup hash,
+  uint32_t queue[7][8][32];          // XCD work queues (workgroup hash,
                                      // workgroup bitmap, wave hash, batch,
-                                     // wave bitmap): one counter per XCD,
-                                     // 128 bytes apart
+                                     // wave bitmap, workgroup pull, wave
+                                     // pull): one counter per XCD, 128 bytes
+                                     // apart
   unsigned long long t[4];           // class counters
-  unsigned long long kern_edges[8];  // oriented edges counted per kernel
-                                     // (merge-lane, merge-wave, wave-hash,
-                                     // workgroup-hash, wave-bitmap,
-                                     // workgroup-bitmap); slot 7: skipped
-  unsigned long long kern_words[8];  // sum of d+(u) + d+(v) over those edges
+  unsigned long long kern_edges[10]; // oriented edges counted per kernel
+                                     // (launch order); slot 8: left out by
+                                     // a partial table, slot 9: skipped
+  unsigned long long kern_words[10]; // sum of d+(u) + d+(v) over those edges
   uint8_t table[kBuckets];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl alignment");
@@ -125,6 +125,7 @@ __device__ uint8_t mlp_rule(int bk, uint32_t cnt, const DispatchParams &p) {
   }
   if (best == 1) return S_HASH;
   if (best == 2) return S_BITMAP;
+  if (best == 3) return S_PULL;
   return merge_variant(bk, p.tune);
 }
 
@@ -136,11 +137,14 @@ __device__ uint8_t decide(int bk, uint32_t cnt, const DispatchParams &p) {
       return S_HASH;
     case D_FORCE_BITMAP:
       return S_BITMAP;
-    case D_TABLE:  // host table: 0 skip, 1 merge, 2 hash, 3 bitmap
+    case D_FORCE_PULL:
+      return S_PULL;
+    case D_TABLE:  // host table: 0 skip, 1 merge, 2 hash, 3 bitmap, 4 pull
       switch (p.host_table[bk]) {
         case 1: return merge_variant(bk, p.tune);
         case 2: return S_HASH;
         case 3: return S_BITMAP;
+        case 4: return S_PULL;
         default: return S_SKIP;
       }
     case D_LEARNED:
@@ -155,19 +159,37 @@ __device__ uint8_t decide(int bk, uint32_t cnt, const DispatchParams &p) {
 // it is short enough for one wave (any span: per-wave window plus at most d
 // low entries), or when its entries below the workgroup window fit kLowCap
 // (at most d - 1 of them lie below the window, none when span <= window).
-__device__ __forceinline__ bool bitmap_fits(const TuneParams &t, uint32_t d, uint32_t span) {
+// span() is evaluated only when the answer depends on it.
+template <class Span>
+__device__ __forceinline__ bool bitmap_fits(const TuneParams &t, uint32_t d, const Span &span) {
   if (d <= kWaveMaxDeg) return true;
   if (d > kBitmapMaxDeg) return false;
-  return span <= (1u << t.bitmap_span_log2) || d - 1u <= kLowCap;
+  return d - 1u <= kLowCap || span() <= (1u << t.bitmap_span_log2);
+}
+
+// A target list fits the pull kernels when it has at most kLowCap entries:
+// all of its entries below any window then fit the low array.  The bucket's
+// target class clog2(db) decides it (db <= 1024 <=> class <= 10), so binning
+// reads no list ends, and every in-edge of one target gets the same answer.
+constexpr int kPullMaxClass = 10;  // clog2(kLowCap)
+static_assert((1u << kPullMaxClass) == kLowCap, "pull fit class");
+
+__device__ __forceinline__ bool pull_fits(const TuneParams &t, int bk) {
+  return t.allow_pull && bk % kBucketSide <= kPullMaxClass;
 }
 
 // Final strategy of an edge in bucket bk: the bin's choice, demoted when the
-// source list (dfull_a entries over span_a ranks) does not fit the LDS
-// structure.  Binning (from the stored bucket) and the vertex-grouped kernels
-// (through edge_strategy) apply the same rule, so both agree edge by edge.
+// list it stages does not fit the LDS structure -- pull (target list) to
+// bitmap, bitmap (source list of dfull_a entries over span_a ranks) to hash,
+// hash to merge.  Binning (from the stored bucket) and the vertex-grouped
+// kernels (through edge_strategy) apply the same rule, so both agree edge by
+// edge.  span_a is read only when the chain reaches the bitmap test (span_of
+// below).
+template <class Span>
 __device__ __forceinline__ uint8_t bucket_strategy(const uint8_t *table, const TuneParams &t,
-                                                   uint32_t dfull_a, uint32_t span_a, int bk) {
+                                                   uint32_t dfull_a, const Span &span_a, int bk) {
   uint8_t s = table[bk];
+  if (s == S_PULL && !pull_fits(t, bk)) s = S_BITMAP;
   if (s == S_BITMAP && !bitmap_fits(t, dfull_a, span_a)) s = S_HASH;
   if (s == S_HASH && dfull_a > (1u << t.hash_max_log2)) s = merge_variant(bk, t);
   return s;
@@ -178,7 +200,7 @@ __device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table,
                                                  uint32_t dfull_a, uint32_t span_a,
                                                  uint32_t da_eff, uint32_t db) {
   if (da_eff == 0 || db == 0) return S_SKIP;
-  return bucket_strategy(table, t, dfull_a, span_a, bucket_of(da_eff, db));
+  return bucket_strategy(table, t, dfull_a, [&] { return span_a; }, bucket_of(da_eff, db));
 }
 
 // Bucket byte of an edge that cannot close a triangle (da = 0 or db = 0).
@@ -306,9 +328,11 @@ __device__ __forceinline__ uint32_t lds_append(bool pred, uint32_t *s_len) {
 
 // Strategy of every edge from its stored bucket byte; merge edges go to the
 // workgroup's list segments, hash and bitmap edges flag their source vertex
-// (bit 0 hash, bit 1 bitmap).  Edges are sorted by source, so the lanes of one
-// source form runs: each run's first lane sets the run's flag bits with one
-// atomic, only when they are not set yet.
+// (bit 0 hash, bit 1 bitmap), pull edges their target vertex (bit 2).  Edges
+// are sorted by source, so the lanes of one source form runs: each run's
+// first lane sets the run's flag bits with one atomic, only when they are not
+// set yet; a pull target's flag is set once, by the first edge that sees it
+// clear.
 __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
                                                      const uint8_t *__restrict__ bk8,
@@ -334,16 +358,18 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
     uint8_t s = S_SKIP;
-    uint32_t a = 0xFFFFFFFFu;
+    uint32_t a = 0xFFFFFFFFu, v = 0;
     if (e < g.m) {
       a = g.src[e];
       const uint32_t bk = bk8[e];
       if (bk != kNoBucket) {
         const uint32_t la = g.off[a], ea = g.off[a + 1];
-        const uint32_t span = table[bk] == S_BITMAP ? span_of(g.col, la, ea) : 0u;
-        s = bucket_strategy(table, tune, ea - la, span, (int)bk);
+        s = bucket_strategy(table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
+                            (int)bk);
+        if (s == S_PULL) v = g.col[e] >> 1;
       }
     }
+    if (s == S_PULL && !(vflag[v] & 4u)) atomicOr(&vflag[v], 4u);  // pull: flag the target
     uint32_t p = lds_append(s == S_MERGE_LANE, &s_len[0]);
     if (s == S_MERGE_LANE) lane_out[p] = (uint32_t)e;
     p = lds_append(s == S_MERGE_WAVE, &s_len[1]);
@@ -390,97 +416,136 @@ __global__ __launch_bounds__(256) void k_list_compact(const uint32_t *__restrict
     ctrl->len[1] = blk_pos[2 * nblk - 1] + cnt - blk_pos[nblk];
 }
 
-// Vertices with hash / bitmap work go to four lists in ascending rank:
-// workgroup hash, workgroup bitmap, wave hash, wave bitmap (sources with at
-// most kWaveMaxDeg out-edges use the wave kernels).  A workgroup owns a tile
-// of kVTile consecutive vertices, visited in rounds of 256: k_vcount counts
-// each list's members per tile, an exclusive scan over the tile counts gives
-// each tile's list offsets, and k_vscatter writes the members in order and
-// clears the flags.
+// Vertices with hash / bitmap / pull work go to six lists in ascending rank:
+// workgroup hash, workgroup bitmap, wave hash, wave bitmap, workgroup pull,
+// wave pull (vertices with at most kWaveMaxDeg out-edges use the wave
+// kernels).  For pull the flagged vertex is the target v whose list is
+// staged, and its entries are chunks of its in-edge range [tin_off[v],
+// tin_off[v + 1]): one entry per kPullBlockChunk (workgroup) or
+// kPullWaveChunk (wave) in-edges, holding the chunk's first position, so a
+// target with a huge in-degree is spread over many workgroups.  A workgroup
+// owns a tile of kVTile consecutive vertices, visited in rounds of 256:
+// k_vcount counts each list's entries per tile, an exclusive scan over the
+// tile counts gives each tile's list offsets, and k_vscatter writes the
+// entries in order and clears the flags.
 constexpr uint32_t kVTile = 4096;
+constexpr int kVLists = 6;
+constexpr uint32_t kPullBlockChunk = 2048;
+constexpr uint32_t kPullWaveChunk = 256;
 
+// Entries of each list a vertex adds (0 or 1, pull lists one per chunk).
 __device__ __forceinline__ void vertex_lists_of(uint32_t v, uint32_t n, const uint32_t *vflag,
-                                                const uint32_t *off, bool c[4], uint32_t &f) {
+                                                const uint32_t *off, const uint32_t *tin_off,
+                                                uint32_t c[kVLists], uint32_t &f) {
   f = 0;
   bool small = false;
+  uint32_t indeg = 0;
   if (v < n) {
     f = vflag[v];
     if (f) small = off[v + 1] - off[v] <= kWaveMaxDeg;
+    if (f & 4u) indeg = tin_off[v + 1] - tin_off[v];
   }
   c[0] = (f & 1u) && !small;
   c[1] = (f & 2u) && !small;
   c[2] = (f & 1u) && small;
   c[3] = (f & 2u) && small;
+  c[4] = (f & 4u) && !small ? (indeg + kPullBlockChunk - 1) / kPullBlockChunk : 0u;
+  c[5] = (f & 4u) && small ? (indeg + kPullWaveChunk - 1) / kPullWaveChunk : 0u;
+}
+
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  return x;
+}
+
+// Upper bound of the entries of one list: every vertex once plus the extra
+// chunks of the pull lists (at most m / kPullWaveChunk).
+__host__ __device__ inline uint64_t vlist_stride(uint32_t n, uint64_t m) {
+  return (uint64_t)n + 1 + m / kPullWaveChunk + 1;
 }
 
 __global__ __launch_bounds__(256) void k_vcount(const uint32_t *__restrict__ vflag,
-                                                const uint32_t *__restrict__ off, uint32_t n,
+                                                const uint32_t *__restrict__ off,
+                                                const uint32_t *__restrict__ tin_off, uint32_t n,
                                                 uint32_t *__restrict__ tile_cnt) {
-  __shared__ uint32_t s_cnt[4];
+  __shared__ uint32_t s_cnt[kVLists];
   const uint32_t tid = threadIdx.x, ntiles = gridDim.x;
-  if (tid < 4) s_cnt[tid] = 0;
+  if (tid < kVLists) s_cnt[tid] = 0;
   __syncthreads();
-  uint32_t cnt[4] = {0, 0, 0, 0};
+  uint32_t cnt[kVLists] = {0, 0, 0, 0, 0, 0};
   const uint32_t base = blockIdx.x * kVTile;
   for (uint32_t r = 0; r < kVTile; r += 256) {
-    bool c[4];
+    uint32_t c[kVLists];
     uint32_t f;
-    vertex_lists_of(base + r + tid, n, vflag, off, c, f);
+    vertex_lists_of(base + r + tid, n, vflag, off, tin_off, c, f);
 #pragma unroll
-    for (int L = 0; L < 4; L++) cnt[L] += (uint32_t)__popcll(__ballot(c[L]));
+    for (int L = 0; L < kVLists; L++) cnt[L] += c[L];
   }
-  if (__lane_id() == 0)
 #pragma unroll
-    for (int L = 0; L < 4; L++) atomicAdd(&s_cnt[L], cnt[L]);
+  for (int L = 0; L < kVLists; L++)
+    if (cnt[L]) atomicAdd(&s_cnt[L], cnt[L]);
   __syncthreads();
-  if (tid < 4) tile_cnt[tid * ntiles + blockIdx.x] = s_cnt[tid];
+  if (tid < kVLists) tile_cnt[tid * ntiles + blockIdx.x] = s_cnt[tid];
 }
 
-// tile_pos: exclusive scan of tile_cnt (list-major, 4 * ntiles entries).
+// tile_pos: exclusive scan of tile_cnt (list-major, kVLists * ntiles
+// entries); list L goes to vlist + L * stride.
 __global__ __launch_bounds__(256) void k_vscatter(uint32_t *__restrict__ vflag,
-                                                  const uint32_t *__restrict__ off, uint32_t n,
+                                                  const uint32_t *__restrict__ off,
+                                                  const uint32_t *__restrict__ tin_off, uint32_t n,
                                                   const uint32_t *__restrict__ tile_cnt,
                                                   const uint32_t *__restrict__ tile_pos,
-                                                  uint32_t *__restrict__ vh,
-                                                  uint32_t *__restrict__ vb,
-                                                  uint32_t *__restrict__ vw,
-                                                  uint32_t *__restrict__ vwb, Ctrl *ctrl) {
-  __shared__ uint32_t s_wcnt[4][4];  // [list][wave] members in this round
-  __shared__ uint32_t s_run[4];      // members of earlier rounds in this tile
+                                                  uint32_t *__restrict__ vlist, uint64_t stride,
+                                                  Ctrl *ctrl) {
+  __shared__ uint32_t s_wcnt[kVLists][4];  // [list][wave] entries in this round
+  __shared__ uint32_t s_run[kVLists];      // entries of earlier rounds in this tile
   const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   const uint32_t ntiles = gridDim.x, b = blockIdx.x;
-  uint32_t *const lists[4] = {vh, vb, vw, vwb};
-  uint32_t tile_off[4];
+  uint32_t tile_off[kVLists];
 #pragma unroll
-  for (int L = 0; L < 4; L++) tile_off[L] = tile_pos[L * ntiles + b] - tile_pos[L * ntiles];
-  if (tid < 4) s_run[tid] = 0;
+  for (int L = 0; L < kVLists; L++)
+    tile_off[L] = tile_pos[L * ntiles + b] - tile_pos[L * ntiles];
+  if (tid < kVLists) s_run[tid] = 0;
   const uint32_t base = b * kVTile;
   for (uint32_t r = 0; r < kVTile; r += 256) {
     const uint32_t v = base + r + tid;
-    bool c[4];
+    uint32_t c[kVLists];
     uint32_t f;
-    vertex_lists_of(v, n, vflag, off, c, f);
-    uint64_t m[4];
+    vertex_lists_of(v, n, vflag, off, tin_off, c, f);
+    uint32_t incl[kVLists];
 #pragma unroll
-    for (int L = 0; L < 4; L++) {
-      m[L] = __ballot(c[L]);
-      if (lane == 0) s_wcnt[L][wave] = (uint32_t)__popcll(m[L]);
+    for (int L = 0; L < kVLists; L++) {
+      incl[L] = wave_scan_incl(c[L]);
+      if (lane == 63) s_wcnt[L][wave] = incl[L];
     }
     __syncthreads();
 #pragma unroll
-    for (int L = 0; L < 4; L++) {
+    for (int L = 0; L < kVLists; L++) {
       if (c[L]) {
-        uint32_t p = tile_off[L] + s_run[L] + (uint32_t)__popcll(m[L] & lanemask_lt());
+        uint32_t p = tile_off[L] + s_run[L] + incl[L] - c[L];
         for (uint32_t w = 0; w < wave; w++) p += s_wcnt[L][w];
-        lists[L][p] = v;
+        uint32_t *dst = vlist + (uint64_t)L * stride + p;
+        if (L < 4) {
+          dst[0] = v;
+        } else {
+          const uint32_t chunk = L == 4 ? kPullBlockChunk : kPullWaveChunk;
+          const uint32_t t0 = tin_off[v];
+          for (uint32_t k = 0; k < c[L]; k++) dst[k] = t0 + k * chunk;
+        }
       }
     }
     if (f) vflag[v] = 0u;
     __syncthreads();
-    if (tid < 4) s_run[tid] += s_wcnt[tid][0] + s_wcnt[tid][1] + s_wcnt[tid][2] + s_wcnt[tid][3];
+    if (tid < kVLists)
+      s_run[tid] += s_wcnt[tid][0] + s_wcnt[tid][1] + s_wcnt[tid][2] + s_wcnt[tid][3];
     __syncthreads();
   }
-  if (b == ntiles - 1 && tid < 4) {
+  if (b == ntiles - 1 && tid < kVLists) {
     const uint32_t last = tid * ntiles + ntiles - 1;
     ctrl->len[2 + tid] = tile_pos[last] + tile_cnt[last] - tile_pos[tid * ntiles];
   }
@@ -491,41 +556,44 @@ __global__ __launch_bounds__(256) void k_vscatter(uint32_t *__restrict__ vflag,
 // list fit, source length for wave or workgroup kernels), with its oriented
 // edge count and d+(u) + d+(v) sum.  Slots follow the launch order
 // (merge-lane, merge-wave, wave-hash, workgroup-hash, wave-bitmap,
-// workgroup-bitmap); slot 6 the edges a partial table leaves out, slot 7 the
+// workgroup-bitmap, wave-pull, workgroup-pull; pull edges by the target
+// list's length); slot 8 the edges a partial table leaves out, slot 9 the
 // edges that cannot close a triangle.
 __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
                                                     const DispatchParams *P,
                                                     const uint8_t *__restrict__ bk8) {
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
-  __shared__ unsigned long long s_e[8], s_w[8];
+  __shared__ unsigned long long s_e[10], s_w[10];
   const int tid = threadIdx.x;
   table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
-  if (tid < 8) s_e[tid] = s_w[tid] = 0;
+  if (tid < 10) s_e[tid] = s_w[tid] = 0;
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < g.m; e += stride) {
     const uint32_t bk = bk8[e];
-    uint32_t slot = 7, words = 0;
+    uint32_t slot = 9, words = 0;
     if (bk != kNoBucket) {
       const uint32_t a = g.src[e], b = g.col[e] >> 1;
       const uint32_t la = g.off[a], ea = g.off[a + 1];
-      const uint32_t span = table[bk] == S_BITMAP ? span_of(g.col, la, ea) : 0u;
-      const uint8_t s = bucket_strategy(table, tune, ea - la, span, (int)bk);
-      const bool small = ea - la <= kWaveMaxDeg;
-      slot = s == S_SKIP         ? 6u  // left out by a partial strategy table
+      const uint32_t db = g.off[b + 1] - g.off[b];
+      const uint8_t s = bucket_strategy(table, tune, ea - la,
+                                        [&] { return span_of(g.col, la, ea); }, (int)bk);
+      const bool small = ea - la <= kWaveMaxDeg, small_b = db <= kWaveMaxDeg;
+      slot = s == S_SKIP         ? 8u  // left out by a partial strategy table
              : s == S_MERGE_LANE ? 0u
              : s == S_MERGE_WAVE ? 1u
              : s == S_HASH       ? (small ? 2u : 3u)
-                                 : (small ? 4u : 5u);
-      words = (ea - la) + (g.off[b + 1] - g.off[b]);
+             : s == S_BITMAP     ? (small ? 4u : 5u)
+                                 : (small_b ? 6u : 7u);
+      words = (ea - la) + db;
     }
     atomicAdd(&s_e[slot], 1ull);
     atomicAdd(&s_w[slot], (unsigned long long)words);
   }
   __syncthreads();
-  if (tid < 8) {
+  if (tid < 10) {
     if (s_e[tid]) atomicAdd(&ctrl->kern_edges[tid], s_e[tid]);
     if (s_w[tid]) atomicAdd(&ctrl->kern_words[tid], s_w[tid]);
   }
@@ -817,7 +885,7 @@ struct BitmapProbe {
     uint32_t w = 0;
     if (d2 < span2) w = B[d2 >> 5];
     uint32_t q = (w >> (d2 & 30u)) & 3u;
-    if (__ballot(y < lo2)) {  // wave-uniform
+    if (nlow && __ballot(y < lo2)) {  // wave-uniform; no low entries: no hit below
       if (y < lo2) q = low_lookup(L, nlow, y >> 1);
     }
     return q;
@@ -847,12 +915,17 @@ struct HashProbe {
   }
 };
 
-// Stream the out-lists N+(v) of the source's out-edges j = first, first +
-// step, ... (below ea) that `keep` accepts against `probe`, adding each
-// edge's triangles to acc.
+// Stream one list per edge index j = first, first + step, ... (below ea)
+// that `keep` accepts against `probe`, adding each edge's triangles to acc.
+// meta(j, w, pb, eb, x) gives edge j's column word w (bit 0: the sign of the
+// edge between the staged vertex and the list's owner), its list col[pb, eb)
+// and the vertex x that owns the edge's triangles when EDGE_OWNER (pull:
+// each triangle's lowest-ranked vertex, credited per edge for per-vertex
+// counts).  Push streams N+(v) for the staged source's out-edges; pull
+// streams the part of N+(u) after v for the staged target's in-edges.
 //   Edge metadata moves 64 edges at a time, one edge per lane: the column
-// word, the target's list bounds and the keep flag are loaded and evaluated
-// in parallel, and the kept edges are visited through the ballot with
+// word, the list bounds and the keep flag are loaded and evaluated in
+// parallel, and the kept edges are visited through the ballot with
 // readlane, so no memory latency sits between consecutive edges.
 //   List entries move in units of up to 64 * kUnroll entries of one list.  The
 // next unit's loads (the rest of the list, or the head of the next kept
@@ -864,9 +937,9 @@ struct HashProbe {
 // The loads are unconditional, their per-lane offsets never change and only
 // the descriptor (SGPRs) moves, so each wait counts exactly the loads it
 // needs and no address register is rewritten under a load in flight.
-template <bool CLS, class Keep, class Probe>
+template <bool CLS, bool EDGE_OWNER, class Meta, class Keep, class Probe>
 __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, uint32_t ea,
-                                             uint32_t step, const Keep &keep,
+                                             uint32_t step, const Meta &meta, const Keep &keep,
                                              const Probe &probe, Acc<CLS> &acc) {
   const uint32_t *__restrict__ col = g.col;
   const uint32_t lane = __lane_id();
@@ -880,9 +953,10 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
       y[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)((u * 64u + lane) * 4u), 0, 0);
   };
   uint32_t e0 = 0, e1 = 0, e2 = 0;
-  // a unit: column word of its edge (bit 0 = sign), first entry, list end
+  // a unit: column word of its edge (bit 0 = sign), first entry, list end,
+  // the vertex owning the edge's triangles (EDGE_OWNER)
   struct Unit {
-    uint32_t w, pos, eb;
+    uint32_t w, pos, eb, x;
   };
   auto lane_of = [](uint32_t v, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)v, i); };
   auto probe_unit = [&](const uint32_t(&y)[kUnroll], const Unit &c) {
@@ -897,39 +971,42 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(y[u]));
     if (c.pos + kUnit >= c.eb) {  // last unit of its list
-      acc.add_edge(c.w & 1u, e0, e1, e2);
+      if constexpr (CLS && EDGE_OWNER) {
+        const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
+        acc.add_edge(c.w & 1u, e0, e1, e2);
+        pv_credit<CLS>(g, c.x, acc, before);
+      } else {
+        acc.add_edge(c.w & 1u, e0, e1, e2);
+      }
       e0 = e1 = e2 = 0;
     }
   };
   for (uint32_t j0 = first; j0 < ea; j0 += 64u * step) {
     const uint32_t j = j0 + lane * step;
-    uint32_t w = 0, pb = 0, eb = 0;
-    if (j < ea) {
-      w = col[j];
-      pb = g.off[w >> 1];
-      eb = g.off[(w >> 1) + 1];
-    }
+    uint32_t w = 0, pb = 0, eb = 0, x = 0;
+    if (j < ea) meta(j, w, pb, eb, x);
     uint64_t mask = __ballot(j < ea && keep(j, pb, eb));
     // next unit after c: the rest of its list, else the next kept edge's list;
     // when none is left, n repeats c past its end (its loads stay in bounds)
     auto advance = [&](const Unit &c, Unit &n) -> bool {
       if (c.pos + kUnit < c.eb) {
-        n = Unit{c.w, c.pos + kUnit, c.eb};
+        n = Unit{c.w, c.pos + kUnit, c.eb, c.x};
         return true;
       }
       if (mask) {
         const int i = __builtin_ctzll(mask);
         mask &= mask - 1;
-        n = Unit{lane_of(w, i), lane_of(pb, i), lane_of(eb, i)};
+        n = Unit{lane_of(w, i), lane_of(pb, i), lane_of(eb, i),
+                 EDGE_OWNER ? lane_of(x, i) : 0u};
         return true;
       }
-      n = Unit{c.w, c.pos + kUnit, c.eb};
+      n = Unit{c.w, c.pos + kUnit, c.eb, c.x};
       return false;
     };
     if (!mask) continue;
     const int i0 = __builtin_ctzll(mask);
     mask &= mask - 1;
-    Unit a{lane_of(w, i0), lane_of(pb, i0), lane_of(eb, i0)};
+    Unit a{lane_of(w, i0), lane_of(pb, i0), lane_of(eb, i0), EDGE_OWNER ? lane_of(x, i0) : 0u};
     uint32_t ya[kUnroll], yb[kUnroll];
     load(ya, a.pos, a.eb);
     // two buffers in turn: the other buffer's loads are in flight while one
@@ -950,6 +1027,155 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
     for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(ya[u]), "v"(yb[u]));
   }
 }
+
+// Inclusive max-scan over the wave.
+__device__ __forceinline__ uint32_t wave_max_scan_incl(uint32_t x) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+    if (lane >= (uint32_t)d) x = max(x, y);
+  }
+  return x;
+}
+
+// Packed streaming, for short lists: the kept lists of a 64-edge metadata
+// batch are laid end to end (exclusive scan P of their lengths) and read in
+// units of kPackUnit consecutive entries, which may span many lists, so a
+// unit's loads are full whatever the list lengths.  Entry f of the packed
+// sequence belongs to the last list i with P_i <= f and lies at col[f + pb_i
+// - P_i].  Per unit, each list starting inside the unit writes i + 1 at its
+// start slot of the wave's heads array H (kPackUnit bytes, one per slot);
+// each lane then carries the last head forward over its 8 consecutive slots
+// and a wave max-scan carries it across lanes, which leaves every slot's
+// list index in H.  Each entry's triangle class follows from the sign bits
+// of its three edges, so no per-list reduction is needed (callers that
+// credit triangles per edge owner use stream_edges).
+constexpr uint32_t kPackUnit = 512;  // 64 lanes x 8 slots
+static_assert(kPackUnit == 64u * kUnroll, "packed unit: 8 slots per lane");
+
+template <bool CLS, class Meta, class Keep, class Probe>
+__device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, uint32_t ea,
+                                              uint32_t step, const Meta &meta, const Keep &keep,
+                                              const Probe &probe, uint64_t *H, Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  uint8_t *Hb = (uint8_t *)H;
+  for (uint32_t j0 = first; j0 < ea; j0 += 64u * step) {
+    const uint32_t j = j0 + lane * step;
+    uint32_t w = 0, pb = 0, eb = 0, x = 0;
+    if (j < ea) meta(j, w, pb, eb, x);
+    const bool kept = j < ea && keep(j, pb, eb);
+    const uint32_t len = kept ? eb - pb : 0u;
+    const uint32_t incl = wave_scan_incl(len);
+    const uint32_t P = incl - len;
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    if (total == 0u) continue;
+    const uint32_t D = pb - P;  // modulo 2^32: f + D = pb + (f - P)
+    const uint64_t negm = __ballot(kept && (w & 1u));
+    // list index of each slot of the unit at B, and its loads (unconditional:
+    // slots past the end read col[0] and are masked when probed)
+    auto prepare = [&](uint32_t B, uint32_t(&y)[kUnroll], uint32_t(&li)[kUnroll],
+                       bool only_used) {
+      wave_lds_sync();  // the previous unit's reads of H are done
+      H[lane] = 0ull;
+      wave_lds_sync();
+      if (len != 0u && P >= B && P - B < kPackUnit) Hb[P - B] = (uint8_t)(lane + 1u);
+      wave_lds_sync();
+      const uint64_t h = H[lane];
+      // the list covering slot B (it may start in an earlier unit)
+      const uint64_t cov = __ballot(len != 0u && P <= B);
+      const uint32_t run_in = cov ? 64u - (uint32_t)__clzll(cov) : 0u;
+      uint32_t last = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) last = max(last, (uint32_t)(h >> (8 * k)) & 0xFFu);
+      const uint32_t scan = wave_max_scan_incl(last);
+      const uint32_t before = (uint32_t)__shfl_up((int)scan, 1, 64);
+      uint32_t run = max(run_in, lane ? before : 0u);
+      uint64_t r = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        run = max(run, (uint32_t)(h >> (8 * k)) & 0xFFu);
+        r |= (uint64_t)run << (8 * k);
+      }
+      wave_lds_sync();
+      H[lane] = r;
+      wave_lds_sync();
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (only_used && B + u * 64u >= total) break;  // wave-uniform
+        const uint32_t f = B + u * 64u + lane;
+        li[u] = (uint32_t)Hb[u * 64u + lane] - 1u;
+        const uint32_t d = (uint32_t)__shfl((int)D, (int)(li[u] & 63u), 64);
+        y[u] = col[f < total ? f + d : 0u];
+      }
+    };
+    auto consume = [&](uint32_t B, const uint32_t(&y)[kUnroll], const uint32_t(&li)[kUnroll]) {
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (B + u * 64u >= total) break;  // wave-uniform
+        const uint32_t yu = B + u * 64u + lane < total ? y[u] : kEmpty;
+        const uint32_t q = probe(yu);
+        const uint32_t neg = (uint32_t)((negm >> (li[u] & 63u)) & 1ull) + (q >> 1) + (yu & 1u);
+        acc.add((q & 1u) != 0u, neg);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(y[u]));
+    };
+    // two units in turn: one unit's loads are in flight while the other is
+    // probed.  The next unit is prepared unconditionally (past the end all
+    // its slots read col[0]), so every wait counts exactly the loads it needs.
+    uint32_t ya[kUnroll], la[kUnroll], yb[kUnroll], lb[kUnroll];
+    uint32_t B = 0;
+    if (total <= kPackUnit) {  // one unit: nothing to overlap, load only its used rounds
+      prepare(B, ya, la, true);
+      consume(B, ya, la);
+      continue;
+    }
+    prepare(B, ya, la, false);
+    while (true) {
+      prepare(B + kPackUnit, yb, lb, false);
+      consume(B, ya, la);
+      B += kPackUnit;
+      if (B >= total) break;
+      prepare(B + kPackUnit, ya, la, false);
+      consume(B, yb, lb);
+      B += kPackUnit;
+      if (B >= total) break;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(ya[u]), "v"(yb[u]));
+  }
+}
+
+// Edge metadata of push streaming: out-edge j of the staged source; its list
+// is the target's out-list N+(v).
+struct PushMeta {
+  const DevCsr &g;
+  __device__ __forceinline__ void operator()(uint32_t j, uint32_t &w, uint32_t &pb, uint32_t &eb,
+                                             uint32_t &x) const {
+    w = g.col[j];
+    pb = g.off[w >> 1];
+    eb = g.off[(w >> 1) + 1];
+    x = 0;
+  }
+};
+
+// Edge metadata of pull streaming: in-edge j of the staged target v is the
+// record of edge e = (u -> v): its list is the part of N+(u) after v,
+// col[e + 1, e + 1 + len), with the edge's sign, in one 8-byte load.  u owns
+// the edge's triangles; it is read (src[e]) only for per-vertex counts.
+struct PullMeta {
+  const DevCsr &g;
+  __device__ __forceinline__ void operator()(uint32_t j, uint32_t &w, uint32_t &pb, uint32_t &eb,
+                                             uint32_t &x) const {
+    const InEdge r = g.tin[j];
+    w = r.len_neg & 1u;
+    pb = r.pos;
+    eb = r.pos + (r.len_neg >> 1);
+    x = g.pv ? g.src[r.pos - 1u] : 0u;
+  }
+};
 
 // LDS hash table of a source list: `cap` = next_pow2(4 d) slots (load
 // <= 1/4), slot = rank & (cap - 1), linear probing.  Out-lists are runs of
@@ -982,8 +1208,8 @@ __device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, u
     while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
   }
   __syncthreads();
-  stream_edges<CLS>(
-      g, la + wave, ea, nwaves,
+  stream_edges<CLS, false>(
+      g, la + wave, ea, nwaves, PushMeta{g},
       [&](uint32_t j, uint32_t pb, uint32_t eb) {
         return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_HASH;
       },
@@ -1016,8 +1242,8 @@ __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, ui
     while (atomicCAS(&T[h], kEmpty, x) != kEmpty) h = (h + 1) & mask;
   }
   wave_lds_sync();
-  stream_edges<CLS>(
-      g, la, ea, 1u,
+  stream_edges<CLS, false>(
+      g, la, ea, 1u, PushMeta{g},
       [&](uint32_t j, uint32_t pb, uint32_t eb) {
         return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_HASH;
       },
@@ -1033,93 +1259,201 @@ __device__ __forceinline__ uint32_t window_base(uint32_t lo, uint32_t hi, uint32
   return hi - lo + 1u > window ? hi - window + 1u : lo;
 }
 
-// Stage N+(a) as a 2-bit (present, negative) LDS bitmap over the rank window
-// [base, hi] (entry r = bits 2r, 2r+1 for rank base + r), with the entries
-// below base (a prefix of the sorted list, at most kLowCap of them) copied to
-// L.  B must be zero on entry and is zero again on return (only the words the
-// vertex touched are cleared).  Called by every thread of the workgroup.
+// A vertex x's out-list N+(x) staged in LDS: a 2-bit (present, negative)
+// bitmap over the rank window [base, hi] (entry r = bits 2r, 2r+1 for rank
+// base + r) and the entries below base (a prefix of the sorted list) in the
+// low array L[0, nlow).
+struct Staged {
+  uint32_t la, ea, d, lo, hi, span, base, nlow;
+  __device__ __forceinline__ BitmapProbe probe(const uint32_t *B, const uint32_t *L) const {
+    return BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow};
+  }
+};
+
+// Workgroup staging (every thread calls it; at most kLowCap low entries).  B
+// must be zero on entry; unstage_block clears the words the list set.
+__device__ __forceinline__ Staged stage_block(const DevCsr &g, uint32_t x, uint32_t *B,
+                                              uint32_t *L, uint32_t *s_nlow, uint32_t window) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+  Staged st;
+  st.la = __builtin_amdgcn_readfirstlane(g.off[x]);
+  st.ea = __builtin_amdgcn_readfirstlane(g.off[x + 1]);
+  st.d = st.ea - st.la;
+  st.lo = __builtin_amdgcn_readfirstlane(col[st.la] >> 1);
+  st.hi = __builtin_amdgcn_readfirstlane(col[st.ea - 1] >> 1);
+  st.span = st.hi - st.lo + 1u;
+  st.base = window_base(st.lo, st.hi, window);
+  for (uint32_t k = tid; k < st.d; k += nthreads) {
+    const uint32_t w = col[st.la + k];
+    const uint32_t r = w >> 1;
+    if (r >= st.base) {
+      const uint32_t o = r - st.base;
+      atomicOr(&B[o >> 4], (1u | ((w & 1u) << 1)) << ((o & 15u) * 2u));
+      if (k == 0) *s_nlow = 0u;
+    } else {
+      L[k] = w;
+      if ((col[st.la + k + 1] >> 1) >= st.base) *s_nlow = k + 1u;  // last low entry (hi >= base)
+    }
+  }
+  __syncthreads();
+  st.nlow = __builtin_amdgcn_readfirstlane(*s_nlow);
+  return st;
+}
+
+__device__ __forceinline__ void unstage_block(const DevCsr &g, const Staged &st, uint32_t *B) {
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < st.d; k += blockDim.x) {
+    const uint32_t r = g.col[st.la + k] >> 1;
+    if (r >= st.base) B[(r - st.base) >> 4] = 0u;
+  }
+  __syncthreads();
+}
+
+// Wave staging of a list of at most kWaveMaxDeg entries in the wave's own B
+// (zero on entry) and L (kWaveMaxDeg words); the lane's entry is kept for
+// unstage_wave.
+struct StagedWave {
+  Staged st;
+  bool in_w;
+  uint32_t o;
+};
+
+__device__ __forceinline__ StagedWave stage_wave(const DevCsr &g, uint32_t x, uint32_t *B,
+                                                 uint32_t *L, uint32_t window) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  StagedWave sw;
+  Staged &st = sw.st;
+  st.la = __builtin_amdgcn_readfirstlane(g.off[x]);
+  st.ea = __builtin_amdgcn_readfirstlane(g.off[x + 1]);
+  st.d = st.ea - st.la;
+  st.lo = __builtin_amdgcn_readfirstlane(col[st.la] >> 1);
+  st.hi = __builtin_amdgcn_readfirstlane(col[st.ea - 1] >> 1);
+  st.span = st.hi - st.lo + 1u;
+  st.base = window_base(st.lo, st.hi, window);
+  const bool mine = lane < st.d;
+  uint32_t w = 0;
+  sw.o = 0;
+  if (mine) {
+    w = col[st.la + lane];
+    sw.o = (w >> 1) - st.base;  // wraps for entries below the window
+  }
+  sw.in_w = mine && (w >> 1) >= st.base;
+  if (sw.in_w) atomicOr(&B[sw.o >> 4], (1u | ((w & 1u) << 1)) << ((sw.o & 15u) * 2u));
+  else if (mine) L[lane] = w;  // the low entries are the lanes [0, nlow)
+  st.nlow = (uint32_t)__popcll(__ballot(mine && !sw.in_w));
+  wave_lds_sync();
+  return sw;
+}
+
+__device__ __forceinline__ void unstage_wave(const StagedWave &sw, uint32_t *B) {
+  wave_lds_sync();
+  if (sw.in_w) B[sw.o >> 4] = 0u;
+  wave_lds_sync();
+}
+
+// Push: stage N+(a) (workgroup: every thread calls it) and count a's bitmap
+// edges by streaming each target's out-list.
 template <bool CLS>
 __device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a, uint32_t *B,
                                                     uint32_t *L, uint32_t *s_nlow,
                                                     const uint8_t *table,
                                                     const TuneParams &tune, Acc<CLS> &acc) {
-  const uint32_t *__restrict__ col = g.col;
-  const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = nthreads >> 6;
-  const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
-  const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t nwaves = blockDim.x >> 6;
   [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane(col[ea - 1] >> 1);
-  const uint32_t span = hi - lo + 1u;
-  const uint32_t base = window_base(lo, hi, 1u << tune.bitmap_span_log2);
-  for (uint32_t k = tid; k < d; k += nthreads) {
-    const uint32_t x = col[la + k];
-    const uint32_t r = x >> 1;
-    if (r >= base) {
-      const uint32_t o = r - base;
-      atomicOr(&B[o >> 4], (1u | ((x & 1u) << 1)) << ((o & 15u) * 2u));
-      if (k == 0) *s_nlow = 0u;
-    } else {
-      L[k] = x;
-      if ((col[la + k + 1] >> 1) >= base) *s_nlow = k + 1u;  // last low entry (hi >= base)
-    }
-  }
-  __syncthreads();
-  const uint32_t nlow = __builtin_amdgcn_readfirstlane(*s_nlow);
-  stream_edges<CLS>(
-      g, la + wave, ea, nwaves,
+  const Staged st = stage_block(g, a, B, L, s_nlow, 1u << tune.bitmap_span_log2);
+  stream_edges<CLS, false>(
+      g, st.la + wave, st.ea, nwaves, PushMeta{g},
       [&](uint32_t j, uint32_t pb, uint32_t eb) {
-        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_BITMAP;
+        return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
       },
-      BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow}, acc);
+      st.probe(B, L), acc);
   pv_credit<CLS>(g, a, acc, before);
-  __syncthreads();
-  for (uint32_t k = tid; k < d; k += nthreads) {
-    const uint32_t r = col[la + k] >> 1;
-    if (r >= base) B[(r - base) >> 4] = 0u;
-  }
-  __syncthreads();
+  unstage_block(g, st, B);
 }
 
-// One wave stages N+(a) (at most kWaveMaxDeg entries) as a 2-bit bitmap over
-// a rank window of `window` entries in its own region B, the entries below
-// the window in L (kWaveMaxDeg words), and counts a's bitmap edges.
+// Push, one wave per source of at most kWaveMaxDeg entries.
 template <bool CLS>
 __device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, uint32_t *B,
                                                    uint32_t *L, uint32_t window,
                                                    const uint8_t *table,
                                                    const TuneParams &tune, Acc<CLS> &acc) {
-  const uint32_t *__restrict__ col = g.col;
-  const uint32_t lane = __lane_id();
-  const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[a]);
-  const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[a + 1]), d = ea - la;
   [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(col[la] >> 1);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane(col[ea - 1] >> 1);
-  const uint32_t span = hi - lo + 1u;
-  const uint32_t base = window_base(lo, hi, window);
-  const bool mine = lane < d;
-  uint32_t x = 0, o = 0;
-  if (mine) {
-    x = col[la + lane];
-    o = (x >> 1) - base;  // wraps for entries below the window
-  }
-  const bool in_w = mine && (x >> 1) >= base;
-  if (in_w) atomicOr(&B[o >> 4], (1u | ((x & 1u) << 1)) << ((o & 15u) * 2u));
-  else if (mine) L[lane] = x;  // the low entries are the lanes [0, nlow)
-  const uint32_t nlow = (uint32_t)__popcll(__ballot(mine && !in_w));
-  wave_lds_sync();
-  stream_edges<CLS>(
-      g, la, ea, 1u,
+  const StagedWave sw = stage_wave(g, a, B, L, window);
+  const Staged &st = sw.st;
+  stream_edges<CLS, false>(
+      g, st.la, st.ea, 1u, PushMeta{g},
       [&](uint32_t j, uint32_t pb, uint32_t eb) {
-        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_BITMAP;
+        return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
       },
-      BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow}, acc);
+      st.probe(B, L), acc);
   pv_credit<CLS>(g, a, acc, before);
-  wave_lds_sync();
-  if (in_w) B[o >> 4] = 0u;
-  wave_lds_sync();
+  unstage_wave(sw, B);
+}
+
+// Pull: stage the target's list N+(v) and count the pull in-edges (u -> v)
+// of one chunk of v's in-edge range by streaming the part of N+(u) after v;
+// each edge's triangles belong to u.  The chunk starts at in-edge position
+// j0 and holds at most `chunk` in-edges; v is the target of its first edge.
+// An in-edge is pull iff its bucket (|N+(u) after v|, d+(v)) says pull (v was
+// listed only if its list fits).
+struct PullChunk {
+  uint32_t v, t0, t1;
+};
+
+__device__ __forceinline__ PullChunk pull_chunk(const DevCsr &g, uint32_t j0, uint32_t chunk) {
+  PullChunk c;
+  c.t0 = j0;
+  const uint32_t e = __builtin_amdgcn_readfirstlane(g.tin[j0].pos) - 1u;
+  c.v = __builtin_amdgcn_readfirstlane(g.col[e] >> 1);
+  c.t1 = min(j0 + chunk, __builtin_amdgcn_readfirstlane(g.tin_off[c.v + 1]));
+  return c;
+}
+
+// Pull streaming.  PACKED: the lists are streamed packed (H: the wave's
+// kPackUnit-byte heads array), which pays off for the short lists of the
+// wave kernel's targets; the workgroup kernel's targets stream lists long
+// enough for the list-at-a-time path, which spends fewer instructions per
+// entry.  Per-vertex counts credit each edge's owner and always take the
+// list-at-a-time path.
+template <bool CLS, bool PACKED, class Probe>
+__device__ __forceinline__ void pull_stream(const DevCsr &g, uint32_t first, uint32_t t1,
+                                            uint32_t step, uint32_t d, const uint8_t *table,
+                                            const Probe &probe, uint64_t *H, Acc<CLS> &acc) {
+  auto keep = [&](uint32_t, uint32_t pb, uint32_t eb) {
+    return eb > pb && table[bucket_of(eb - pb, d)] == S_PULL;
+  };
+  if (!PACKED || (CLS && g.pv))
+    stream_edges<CLS, true>(g, first, t1, step, PullMeta{g}, keep, probe, acc);
+  else
+    stream_packed<CLS>(g, first, t1, step, PullMeta{g}, keep, probe, H, acc);
+}
+
+template <bool CLS>
+__device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, uint32_t *B,
+                                                  uint32_t *L, uint32_t *s_nlow,
+                                                  const uint8_t *table, const TuneParams &tune,
+                                                  Acc<CLS> &acc) {
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t nwaves = blockDim.x >> 6;
+  const PullChunk c = pull_chunk(g, j0, kPullBlockChunk);
+  const Staged st = stage_block(g, c.v, B, L, s_nlow, 1u << tune.bitmap_span_log2);
+  pull_stream<CLS, false>(g, c.t0 + wave, c.t1, nwaves, st.d, table, st.probe(B, L), nullptr,
+                          acc);
+  unstage_block(g, st, B);
+}
+
+template <bool CLS>
+__device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t *B,
+                                                 uint32_t *L, uint32_t window, uint64_t *H,
+                                                 const uint8_t *table, Acc<CLS> &acc) {
+  const PullChunk c = pull_chunk(g, j0, kPullWaveChunk);
+  const StagedWave sw = stage_wave(g, c.v, B, L, window);
+  const Staged &st = sw.st;
+  pull_stream<CLS, true>(g, c.t0, c.t1, 1u, st.d, table, st.probe(B, L), H, acc);
+  unstage_wave(sw, B);
 }
 
 // Pop one item for the whole workgroup from an XCD queue.
@@ -1192,9 +1526,11 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   acc.flush(ctrl);
 }
 
-// One workgroup per source vertex: LDS bitmap (zeroed once per workgroup)
-// followed by the kLowCap-word low array.
-template <bool CLS>
+// One workgroup per vertex: LDS bitmap (zeroed once per workgroup) followed
+// by the kLowCap-word low array.  Push (PULL = false) stages sources with
+// more than kWaveMaxDeg out-edges; a pull item is a chunk of the in-edges of
+// such a target (the list entry is the chunk's first in-edge position).
+template <bool CLS, bool PULL>
 __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                          const DispatchParams *P,
                                                          const uint32_t *__restrict__ vlist,
@@ -1204,23 +1540,28 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
-  const uint32_t len = ctrl->len[3];
-  if (len == 0) return;  // the whole grid: no vertex has workgroup-bitmap edges
+  const uint32_t len = ctrl->len[PULL ? 6 : 3];
+  if (len == 0) return;  // the whole grid: no vertex has this kernel's edges
   if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
   for (uint32_t k = threadIdx.x; k < bitmap_words; k += blockDim.x) B[k] = 0u;
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   uint32_t first;
-  while (block_pop(ctrl->queue[1], len, cur, &s_first, &s_count, first))
-    bitmap_vertex_block<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first]), B, L, &s_nlow,
-                             table, tune, acc);
+  while (block_pop(ctrl->queue[PULL ? 5 : 1], len, cur, &s_first, &s_count, first)) {
+    const uint32_t x = __builtin_amdgcn_readfirstlane(vlist[first]);
+    if constexpr (PULL)
+      pull_vertex_block<CLS>(g, x, B, L, &s_nlow, table, tune, acc);
+    else
+      bitmap_vertex_block<CLS>(g, x, B, L, &s_nlow, table, tune, acc);
+  }
   acc.flush(ctrl);
 }
 
-// One wave per source vertex (source lists of at most kWaveMaxDeg entries):
-// each wave owns window_words bitmap words followed by kWaveMaxDeg low words.
-template <bool CLS>
+// One wave per vertex (lists of at most kWaveMaxDeg entries): each wave owns
+// window_words bitmap words followed by kWaveMaxDeg low words.  Push stages
+// sources, pull the targets of in-edge chunks.
+template <bool CLS, bool PULL>
 __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
                                                      const uint32_t *__restrict__ vlist,
@@ -1228,8 +1569,9 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   extern __shared__ uint32_t lds[];
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
-  const uint32_t len = ctrl->len[5];
-  if (len == 0) return;  // the whole grid: no vertex has wave-bitmap edges
+  __shared__ uint64_t s_heads[PULL ? 256 : 1];  // packed streaming, 64 per wave
+  const uint32_t len = ctrl->len[PULL ? 7 : 5];
+  if (len == 0) return;  // the whole grid: no vertex has this kernel's edges
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = __lane_id();
   table[threadIdx.x] = ctrl->table[threadIdx.x];
@@ -1243,13 +1585,18 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   Acc<CLS> acc;
   while (true) {
     uint32_t first = 0, count = 0;
-    if (lane == 0 && !queue_pop(ctrl->queue[4], len, kWaveChunk, cur, first, count)) count = 0;
+    if (lane == 0 && !queue_pop(ctrl->queue[PULL ? 6 : 4], len, kWaveChunk, cur, first, count))
+      count = 0;
     first = __shfl(first, 0, 64);
     count = __shfl(count, 0, 64);
     if (count == 0) break;
-    for (uint32_t it = 0; it < count; it++)
-      bitmap_vertex_wave<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first + it]), B, L,
-                              window, table, tune, acc);
+    for (uint32_t it = 0; it < count; it++) {
+      const uint32_t x = __builtin_amdgcn_readfirstlane(vlist[first + it]);
+      if constexpr (PULL)
+        pull_vertex_wave<CLS>(g, x, B, L, window, s_heads + wave * 64u, table, acc);
+      else
+        bitmap_vertex_wave<CLS>(g, x, B, L, window, table, tune, acc);
+    }
   }
   acc.flush(ctrl);
 }
@@ -1417,9 +1764,11 @@ size_t wave_bitmap_lds_bytes(const TuneParams &t) {
 hipError_t init_kernel_attributes() {
   static bool done = false;
   if (done) return hipSuccess;
-  const void *fns[] = {(const void *)k_bitmap<true>,      (const void *)k_bitmap<false>,
-                       (const void *)k_bitmap_wave<true>, (const void *)k_bitmap_wave<false>,
-                       (const void *)k_hash<true>,        (const void *)k_hash<false>,
+  const void *fns[] = {(const void *)k_bitmap<true, false>,      (const void *)k_bitmap<false, false>,
+                       (const void *)k_bitmap<true, true>,       (const void *)k_bitmap<false, true>,
+                       (const void *)k_bitmap_wave<true, false>, (const void *)k_bitmap_wave<false, false>,
+                       (const void *)k_bitmap_wave<true, true>,  (const void *)k_bitmap_wave<false, true>,
+                       (const void *)k_hash<true>,               (const void *)k_hash<false>,
                        (const void *)k_batch};
   for (const void *f : fns)
     STC_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
@@ -1430,12 +1779,12 @@ hipError_t init_kernel_attributes() {
 namespace {
 
 struct BinOut {
-  uint32_t *lane_list, *wave_list, *vh, *vb, *vw, *vwb;
+  uint32_t *lane_list, *wave_list, *vh, *vb, *vw, *vwb, *vp, *vwp;
 };
 
 // Buffers: cb.gcount holds Ctrl; cb.lists the per-workgroup segments and the
 // packed merge-lane / merge-wave edge lists; cb.vflag n words kept zero
-// between runs; cb.vlist the four vertex lists (n + 1 words each) and the
+// between runs; cb.vlist the six vertex lists (vlist_stride words each) and the
 // per-tile list counts and offsets.  Every buffer is sized before the first launch, so a second call
 // with the same graph allocates nothing (and can be captured in a graph).
 hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
@@ -1458,7 +1807,8 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
     STC_TRY(hipGetLastError());
   }
   const uint32_t ntiles = (uint32_t)(((uint64_t)n + kVTile - 1) / kVTile);
-  STC_TRY(cb.vlist.ensure((4 * ((uint64_t)n + 1) + 8 * (uint64_t)ntiles + 8) * 4));
+  const uint64_t vstride = vlist_stride(n, m);
+  STC_TRY(cb.vlist.ensure((kVLists * vstride + 2 * kVLists * (uint64_t)ntiles + 8) * 4));
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
   uint32_t *lane_seg = cb.lists.as<uint32_t>();
   uint32_t *wave_seg = lane_seg + seg_words;
@@ -1468,11 +1818,13 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   uint32_t *blk_pos = blk_cnt + 2 * bin_grid;
   uint32_t *vf = cb.vflag.as<uint32_t>();
   out.vh = cb.vlist.as<uint32_t>();
-  out.vb = out.vh + (n + 1);
-  out.vw = out.vb + (n + 1);
-  out.vwb = out.vw + (n + 1);
-  uint32_t *tile_cnt = out.vwb + (n + 1);
-  uint32_t *tile_pos = tile_cnt + 4 * (uint64_t)ntiles;
+  out.vb = out.vh + vstride;
+  out.vw = out.vb + vstride;
+  out.vwb = out.vw + vstride;
+  out.vp = out.vwb + vstride;
+  out.vwp = out.vp + vstride;
+  uint32_t *tile_cnt = out.vwp + vstride;
+  uint32_t *tile_pos = tile_cnt + kVLists * (uint64_t)ntiles;
 
   if (marks) STC_TRY(hipEventRecord(marks[0], st));
   hipLaunchKernelGGL(k_fill_words, dim3(8), dim3(256), 0, st, (uint32_t *)ctrl,
@@ -1490,11 +1842,12 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
                      ctrl);
   STC_TRY(hipGetLastError());
   if (n > 0) {
-    hipLaunchKernelGGL(k_vcount, dim3(ntiles), dim3(256), 0, st, vf, g.off, n, tile_cnt);
+    hipLaunchKernelGGL(k_vcount, dim3(ntiles), dim3(256), 0, st, vf, g.off, g.tin_off, n,
+                       tile_cnt);
     STC_TRY(hipGetLastError());
-    STC_TRY(scan_exclusive_u32(tile_cnt, tile_pos, 4 * (uint64_t)ntiles, cb.scan, st));
-    hipLaunchKernelGGL(k_vscatter, dim3(ntiles), dim3(256), 0, st, vf, g.off, n, tile_cnt,
-                       tile_pos, out.vh, out.vb, out.vw, out.vwb, ctrl);
+    STC_TRY(scan_exclusive_u32(tile_cnt, tile_pos, kVLists * (uint64_t)ntiles, cb.scan, st));
+    hipLaunchKernelGGL(k_vscatter, dim3(ntiles), dim3(256), 0, st, vf, g.off, g.tin_off, n,
+                       tile_cnt, tile_pos, out.vh, vstride, ctrl);
     STC_TRY(hipGetLastError());
   }
   if (marks) STC_TRY(hipEventRecord(marks[1], st));
@@ -1536,12 +1889,18 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL(k_hash<true>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
                        dp, L.vh);
     STC_TRY(mark(5));
-    hipLaunchKernelGGL(k_bitmap_wave<true>, dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl, dp,
-                       L.vwb, bw_words);
+    hipLaunchKernelGGL((k_bitmap_wave<true, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
+                       dp, L.vwb, bw_words);
     STC_TRY(mark(6));
-    hipLaunchKernelGGL(k_bitmap<true>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
-                       dp, L.vb, bm_words);
+    hipLaunchKernelGGL((k_bitmap<true, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
+                       ctrl, dp, L.vb, bm_words);
     STC_TRY(mark(7));
+    hipLaunchKernelGGL((k_bitmap_wave<true, true>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
+                       dp, L.vwp, bw_words);
+    STC_TRY(mark(8));
+    hipLaunchKernelGGL((k_bitmap<true, true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
+                       ctrl, dp, L.vp, bm_words);
+    STC_TRY(mark(9));
   } else {
     hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, L.lane_list);
     STC_TRY(mark(2));
@@ -1552,12 +1911,18 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL(k_hash<false>, dim3(hash_grid), dim3(kVertexBlock), hash_lds, st, g, ctrl,
                        dp, L.vh);
     STC_TRY(mark(5));
-    hipLaunchKernelGGL(k_bitmap_wave<false>, dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl, dp,
-                       L.vwb, bw_words);
+    hipLaunchKernelGGL((k_bitmap_wave<false, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
+                       dp, L.vwb, bw_words);
     STC_TRY(mark(6));
-    hipLaunchKernelGGL(k_bitmap<false>, dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
-                       dp, L.vb, bm_words);
+    hipLaunchKernelGGL((k_bitmap<false, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
+                       ctrl, dp, L.vb, bm_words);
     STC_TRY(mark(7));
+    hipLaunchKernelGGL((k_bitmap_wave<false, true>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
+                       dp, L.vwp, bw_words);
+    STC_TRY(mark(8));
+    hipLaunchKernelGGL((k_bitmap<false, true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
+                       ctrl, dp, L.vp, bm_words);
+    STC_TRY(mark(9));
   }
   return hipGetLastError();
 }
@@ -1612,7 +1977,7 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_
   STC_TRY(hipStreamSynchronize(st));
   for (int k = 0; k < 4; k++) r.t[k] = h.t[k];
   (void)classes;
-  for (int k = 0; k < 8; k++) {
+  for (int k = 0; k < 10; k++) {
     r.kern_edges[k] = h.kern_edges[k];
     r.kern_words[k] = h.kern_words[k];
   }

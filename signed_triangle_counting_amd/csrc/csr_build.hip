@@ -327,6 +327,46 @@ __global__ void k_split_csr(const uint64_t *__restrict__ key2, uint64_t m,
   }
 }
 
+// Keys of the transposed index: (target rank, edge id); sorted on the target
+// bits (stably, so edge ids stay ascending within a target).
+__global__ void k_tkeys(const uint32_t *__restrict__ col, uint64_t m, uint64_t *__restrict__ key) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    key[i] = ((uint64_t)(col[i] >> 1) << 32) | i;
+}
+
+// In-edge records of the sorted keys (edge e = (u -> v): the part of N+(u)
+// after v and the edge's sign), and in-degrees (targets are sorted: one
+// atomic per run of equal targets in a wave).
+__global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m,
+                         const uint32_t *__restrict__ col, const uint32_t *__restrict__ src,
+                         const uint32_t *__restrict__ off, InEdge *__restrict__ tin,
+                         uint32_t *__restrict__ tdeg) {
+  const uint32_t lane = __lane_id();
+  const uint64_t mround = (m + 63) & ~63ull;  // keep waves converged
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mround;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const bool in = i < m;
+    uint32_t v = 0xFFFFFFFFu;
+    if (in) {
+      const uint64_t k = key[i];
+      v = (uint32_t)(k >> 32);
+      const uint32_t e = (uint32_t)k;
+      const uint32_t end = off[src[e] + 1];
+      tin[i] = InEdge{e + 1u, ((end - e - 1u) << 1) | (col[e] & 1u)};
+    }
+    const uint32_t prev = (uint32_t)__shfl_up((int)v, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || prev != v);
+    const uint64_t valid = __ballot(in);
+    if (in && ((heads >> lane) & 1u)) {
+      const uint64_t above = heads & ~((2ull << lane) - 1ull);
+      const uint64_t run = (above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull) &
+                           ~((1ull << lane) - 1ull);
+      atomicAdd(&tdeg[v], (uint32_t)__popcll(valid & run));
+    }
+  }
+}
+
 // Algorithmic bytes of the merge model and the maximum out-degree.
 __global__ void k_balg(const uint32_t *__restrict__ off,
                        const uint32_t *__restrict__ col, const uint32_t *__restrict__ src,
@@ -585,6 +625,19 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
                      bb.col.as<uint32_t>(), bb.src.as<uint32_t>(), fdeg);
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(fdeg, bb.off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
+  // transposed index for the pull strategy: in-edge records grouped by target rank
+  STC_TRY(bb.tin.ensure((m ? m : 1) * sizeof(InEdge)));
+  STC_TRY(bb.tin_off.ensure(((uint64_t)n + 1) * 4));
+  hipLaunchKernelGGL(k_tkeys, dim3(grid_for(m, T)), dim3(T), 0, st, bb.col.as<uint32_t>(), m, kA);
+  STC_TRY(hipGetLastError());
+  STC_TRY(radix_sort_u64(&kA, &kB, m, shifts_for(32, nb), bb.hist, bb.scr, st));
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1, T)), dim3(T), 0, st, deg, (uint64_t)n + 1,
+                     0u);
+  hipLaunchKernelGGL(k_tsplit, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m,
+                     bb.col.as<uint32_t>(), bb.src.as<uint32_t>(), bb.off.as<uint32_t>(),
+                     bb.tin.as<InEdge>(), deg);
+  STC_TRY(hipGetLastError());
+  STC_TRY(scan_exclusive_u32(deg, bb.tin_off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
   // algorithmic bytes and max out-degree
   hipLaunchKernelGGL(k_balg, dim3(grid_for(m, T, 8192)), dim3(T), 0, st,
                      bb.off.as<uint32_t>(), bb.col.as<uint32_t>(), bb.src.as<uint32_t>(),

@@ -22,7 +22,9 @@ enum Strategy : uint8_t {
   S_HASH = 3,        // LDS hash table of the source list
   S_BITMAP = 4,      // LDS presence/sign bitmap over a rank window at the top
                      // of the source list (lower part kept as a sorted array)
-  S_COUNT = 5
+  S_PULL = 5,        // the same bitmap of the target's list N+(v), probed by the
+                     // part of N+(u) after v (grouped by target vertex)
+  S_COUNT = 6
 };
 
 // Bucket of an oriented edge (u->v): (ceil_log2 of the usable part of
@@ -35,7 +37,7 @@ constexpr int kBuckets = kBucketSide * kBucketSide;
 // Dispatcher inputs/outputs resident in device memory.
 constexpr int kMlpIn = 7;  // la, lb, la - lb, log2 count, log2 n, log2 m, log2 graphs
 constexpr int kMlpHidden = 16;
-constexpr int kMlpOut = 3;  // merge, hash, bitmap
+constexpr int kMlpOut = 4;  // merge, hash, bitmap, pull
 struct MlpModel {
   float mean[kMlpIn];
   float inv_std[kMlpIn];
@@ -51,7 +53,8 @@ enum DispatchMode : int {
   D_FORCE_MERGE = 2,
   D_FORCE_HASH = 3,
   D_FORCE_BITMAP = 4,
-  D_TABLE = 5  // use the strategy table given by the host (training/timing)
+  D_FORCE_PULL = 5,
+  D_TABLE = 6  // use the strategy table given by the host (training/timing)
 };
 
 struct TuneParams {
@@ -64,6 +67,8 @@ struct TuneParams {
   int hash_block;         // threads per workgroup for hash/bitmap
   int fixed_skew_log2;    // fixed rule: skew >= 2^x -> hash
   int fixed_hub_log2;     // fixed rule: d+(v) >= 2^x (or source) -> bitmap
+  int allow_pull;         // set per call: 0 where no kernel counts pull edges
+                          // (the batched kernel), which then fall back to bitmap
 };
 
 struct DispatchParams {
@@ -76,6 +81,13 @@ struct DispatchParams {
                                  // counted together (k = 1: one graph)
 };
 
+// One in-edge e = (u -> v) of the transposed index: the part of N+(u) after
+// v is col[pos, pos + (len_neg >> 1)) with pos = e + 1; bit 0 of len_neg is
+// the edge's sign.
+struct InEdge {
+  uint32_t pos, len_neg;
+};
+
 // Device view of the oriented CSR (rank space).
 struct DevCsr {
   uint32_t n = 0;
@@ -83,6 +95,8 @@ struct DevCsr {
   const uint32_t *off = nullptr;  // n + 1
   const uint32_t *col = nullptr;  // m: (dst_rank << 1) | neg
   const uint32_t *src = nullptr;  // m: src rank
+  const uint32_t *tin_off = nullptr;  // n + 1: in-edges of each target rank
+  const InEdge *tin = nullptr;        // m: in-edges grouped by target, ascending e
   // Per-vertex four-class counters by rank (4 n words: T0..T3), credited to
   // each triangle's lowest-ranked vertex by the four-class kernels; nullptr
   // when per-vertex counts are not requested.
@@ -138,7 +152,7 @@ struct BuildBuffers {
   DevBuf in_u, in_v, in_s;
   DevBuf keyA, keyB, flag, pos;
   DevBuf deg, vkeyA, vkeyB, rank, perm, graph_of;
-  DevBuf col, src, off, fdeg;
+  DevBuf col, src, off, fdeg, tin, tin_off;
   DevBuf hist, misc;
   ScanScratch scr;
 };
@@ -172,10 +186,11 @@ struct CountBuffers {
 struct CountResult {
   uint64_t t[4] = {0, 0, 0, 0};  // t0..t3 when classes, else t[0]=bal t[1]=unbal
   // per counting kernel (merge-lane, merge-wave, wave-hash, workgroup-hash,
-  // wave-bitmap, workgroup-bitmap; slot 7: skipped edges): oriented edges and
+  // wave-bitmap, workgroup-bitmap, wave-pull, workgroup-pull; slot 8: edges
+  // a partial table leaves out, slot 9: skipped edges): oriented edges and
   // their sum of d+(u) + d+(v)
-  uint64_t kern_edges[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint64_t kern_words[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t kern_edges[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t kern_words[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint8_t table[kBuckets];
   uint32_t bucket_cnt[kBuckets];
 };
@@ -183,10 +198,10 @@ struct CountResult {
 // Enqueue binning + counting on `st` (no host sync inside).  hp is the host
 // copy of *dp (launch shapes come from its tune part).  marks (nullable, 8
 // events) are recorded before binning, after binning and after each of the
-// merge-lane, merge-wave, wave-hash, workgroup-hash, wave-bitmap and
-// workgroup-bitmap kernels.
-constexpr int kMarks = 8;
-constexpr int kKernels = 6;
+// merge-lane, merge-wave, wave-hash, workgroup-hash, wave-bitmap,
+// workgroup-bitmap, wave-pull and workgroup-pull kernels.
+constexpr int kMarks = 10;
+constexpr int kKernels = 8;
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
                          int num_cus, hipStream_t st, const hipEvent_t *marks);

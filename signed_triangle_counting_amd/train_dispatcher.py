@@ -7,9 +7,10 @@ workloads), and for training batches of small graphs counted together by the
 batched kernel, the counter's own binning pass gives the oriented-edge
 histogram over the 256 buckets.  One coordinate-descent pass over the buckets then
 times the whole counting phase with each bucket under each strategy (merge,
-LDS hash, hub bitmap) while every other bucket keeps its current best choice;
-those three in-context times are the bucket's costs.  The samples (bucket
-features, three costs) train the 7-16-3 MLP with the device training kernel
+LDS hash, source-staged bitmap, target-staged pull) while every other bucket
+keeps its current best choice; those four in-context times are the bucket's
+costs.  The samples (bucket
+features, four costs) train the 7-16-4 MLP with the device training kernel
 (stc_train_dispatcher).  The model can only change which kernel counts an
 edge: the evaluation here checks that every dispatch mode returns the same
 counts.
@@ -27,7 +28,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 import signed_triangle_counting_amd as stc  # noqa: E402
 
-STRATS = ("merge", "hash", "bitmap")
+STRATS = ("merge", "hash", "bitmap", "pull")
 np.seterr(all="ignore")
 
 
@@ -77,7 +78,7 @@ def collect(ctx, name, g, reps=3, log=print, batch=False):
     Start from the best single-strategy table; visit buckets heaviest first
     (edges x 2^(a+b)); for each, time the whole counting phase with that
     bucket under each strategy and every other bucket at its current choice,
-    keep the fastest, and record the three times as the bucket's costs.
+    keep the fastest, and record the four times as the bucket's costs.
     Per-vertex staging is shared by a vertex's buckets, so costs measured in
     context, not bucket by bucket in isolation, are what the choice changes.
     """
@@ -126,7 +127,7 @@ def evaluate(ctx, graphs, log=print, batch=False):
         else:
             ctx.prepare(g)
         res = {}
-        for mode in ("fixed", "learned", "merge", "hash", "bitmap"):
+        for mode in ("fixed", "learned", "merge", "hash", "bitmap", "pull"):
             if batch:
                 cnt = tuple(map(tuple, ctx.count_batch_prepared(dispatch=mode, repeat=5)))
             else:
@@ -140,8 +141,8 @@ def evaluate(ctx, graphs, log=print, batch=False):
         row["graph"] = name
         row["learned_vs_fixed"] = round(res["fixed"][0] / res["learned"][0], 3)
         rows.append(row)
-        log("  %-14s fixed %.3f learned %.3f merge %.3f hash %.3f bitmap %.3f  (x%.2f)" % (
-            name, row["fixed"], row["learned"], row["merge"], row["hash"], row["bitmap"],
+        log("  %-14s fixed %.3f learned %.3f merge %.3f hash %.3f bitmap %.3f pull %.3f  (x%.2f)" % (
+            name, row["fixed"], row["learned"], row["merge"], row["hash"], row["bitmap"], row["pull"],
             row["learned_vs_fixed"]))
     return rows
 

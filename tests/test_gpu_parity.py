@@ -14,7 +14,7 @@ import signed_triangle_counting_amd as stc
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["learned", "fixed", "merge", "hash", "bitmap"]
+MODES = ["learned", "fixed", "merge", "hash", "bitmap", "pull"]
 
 
 def say(*a):
@@ -273,7 +273,7 @@ def test_partial_tables_strategy_invariant(ctx, scale, ef):
     total = (0, 0)
     for bs in sets:
         res = {}
-        for s in ("merge", "hash", "bitmap"):
+        for s in ("merge", "hash", "bitmap", "pull"):
             t = np.zeros(256, np.uint8)
             t[bs] = stc.TABLE[s]
             res[s] = ctx.count_with_table(t)
@@ -285,6 +285,26 @@ def test_partial_tables_strategy_invariant(ctx, scale, ef):
         c = ctx.count_with_table(t)
         total = (total[0] + c[0], total[1] + c[1])
     assert total == oracle_counts(g.n, g.u, g.v, g.sign)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_mixed_strategy_tables(ctx, seed):
+    """A table mixing merge, hash, bitmap and pull bucket by bucket counts
+    every triangle exactly once: binning and each kernel's edge test agree
+    on every edge, also where push and pull edges share a vertex and where a
+    target's in-edges span many pull chunks (hub graph)."""
+    rng = np.random.default_rng(seed)
+    graphs = [stc.gen_rmat(17, 16, 0.57, 0.19, 0.19, 0.3, 500 + seed),
+              stc.gen_hub(200000, 1000000, 8, 1.2, 0.6, 2000, 0.3, 600 + seed)]
+    for g in graphs:
+        ctx.prepare(g)
+        want = oracle_counts(g.n, g.u, g.v, g.sign)
+        for _ in range(3):
+            t = rng.choice([stc.TABLE[s] for s in ("merge", "hash", "bitmap", "pull")],
+                           256).astype(np.uint8)
+            got = ctx.count_with_table(t)
+            say("mixed", seed, g.n, got)
+            assert got == want
 
 
 def test_kernel_statistics_account_for_every_edge(ctx):
@@ -310,8 +330,8 @@ def test_kernel_statistics_account_for_every_edge(ctx):
         ctx.count_prepared(dispatch=mode)
         st = ctx.stats()
         say("stats", mode, st["edges_merge"], st["edges_hash"], st["edges_bitmap"],
-            st["edges_skipped"])
-        assert (st["edges_merge"] + st["edges_hash"] + st["edges_bitmap"] +
+            st["edges_pull"], st["edges_skipped"])
+        assert (st["edges_merge"] + st["edges_hash"] + st["edges_bitmap"] + st["edges_pull"] +
                 st["edges_skipped"]) == st["m_oriented"] == o.m
         assert st["edges_skipped"] == int((~counted).sum())
         assert sum(st["bytes_kernel"]) == 4 * words
@@ -332,7 +352,7 @@ def test_batch_two_phase_and_training_hooks(ctx):
     st = ctx.stats()
     assert int(hist.sum()) + st["edges_skipped"] == st["m_oriented"]
     nz = np.nonzero(hist)[0]
-    for s in ("merge", "hash", "bitmap"):
+    for s in ("merge", "hash", "bitmap", "pull"):
         t = np.zeros(256, np.uint8)
         t[nz] = stc.TABLE[s]
         say("batch table", s)
