@@ -672,22 +672,29 @@ __device__ __forceinline__ void pv_credit(const DevCsr &g, uint32_t a, const Acc
   }
 }
 
-// Per-lane probe result folded into wave-uniform edge totals.  q: bit 0 =
-// found, bit 1 = neg(u,x); nb = neg(v,x).
+// Per-lane probe result folded into wave-uniform edge totals.  A probe
+// reports a streamed entry x (sign nb of its edge) as found in the staged
+// list with the same sign (same: the two edges to x carry 0 or 2 negatives,
+// 2 nb of them) or the opposite sign (diff: exactly 1 negative).
 template <bool CLS>
-__device__ __forceinline__ void tally(uint32_t q, uint32_t nb, uint32_t &e0, uint32_t &e1,
-                                      uint32_t &e2) {
+__device__ __forceinline__ void tally(bool same, bool diff, uint32_t nb, uint32_t &e0,
+                                      uint32_t &e1, uint32_t &e2) {
   if (CLS) {
-    const uint32_t n2 = (q >> 1) + nb;
-    const bool f = (q & 1u) != 0u;
-    e0 += (uint32_t)__popcll(__ballot(f && n2 == 0u));
-    e1 += (uint32_t)__popcll(__ballot(f && n2 == 1u));
-    e2 += (uint32_t)__popcll(__ballot(f && n2 == 2u));
+    e0 += (uint32_t)__popcll(__ballot(same && nb == 0u));
+    e1 += (uint32_t)__popcll(__ballot(diff));
+    e2 += (uint32_t)__popcll(__ballot(same && nb != 0u));
   } else {
-    const uint32_t v = q ^ (nb << 1);  // 1: found, even; 3: found, odd
-    e0 += (uint32_t)__popcll(__ballot(v == 1u));
-    e1 += (uint32_t)__popcll(__ballot(v == 3u));
+    e0 += (uint32_t)__popcll(__ballot(same));  // even
+    e1 += (uint32_t)__popcll(__ballot(diff));  // odd
   }
+}
+
+// same / diff of a found staged word x ((rank << 1) | neg) against a
+// streamed entry with sign nb.
+__device__ __forceinline__ void found_as(bool found, uint32_t x, uint32_t nb, bool &same,
+                                         bool &diff) {
+  same = found && ((x ^ nb) & 1u) == 0u;
+  diff = found && ((x ^ nb) & 1u) != 0u;
 }
 
 // ---- merge: one lane per edge ----------------------------------------------
@@ -850,7 +857,7 @@ constexpr int kUnroll = 8;  // 64-element loads per unit (a wave keeps two units
 constexpr int kBufferWord3 = 0x00020000;
 
 // Look up rank yr among the nlow sorted words L[0, nlow) ((rank << 1) | neg)
-// by a branch-free lower bound: bit 0 of the result = found, bit 1 = neg.
+// by a branch-free lower bound: the matching word, or kEmpty.
 __device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, uint32_t nlow,
                                                uint32_t yr) {
   uint32_t pos = 0;  // entries below yr seen so far
@@ -858,37 +865,52 @@ __device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, u
     if (pos + step <= nlow && (L[pos + step - 1] >> 1) < yr) pos += step;
   if (pos < nlow) {
     const uint32_t x = L[pos];
-    if ((x >> 1) == yr) return 1u | ((x & 1u) << 1);
+    if ((x >> 1) == yr) return x;
   }
-  return 0u;
+  return kEmpty;
 }
 
-// ---- probes of a staged source list -------------------------------------------
-// Each returns, per lane, bit 0 = target rank found in N+(u), bit 1 = neg(u,x).
+// ---- probes of a staged list ----------------------------------------------------
+// probe(y, same, diff): whether the streamed entry y = (rank << 1) | nb is in
+// the staged list with the same sign (same) or the opposite sign (diff).
+// y = 0 and y = kEmpty always miss.  0 is no streamed entry (every streamed
+// rank lies above the vertex whose list holds it), and the streaming loops
+// leave the lanes of a buffer load past the list end, which read 0, unmasked.
 
-// LDS 2-bit bitmap B over the rank window [lo, lo + span) of N+(u) (entry r =
-// bits 2r, 2r+1 for rank lo + r) and the sorted array L[0, nlow) of the
-// entries below lo.  With d2 = y - 2 lo = 2 (rank - lo) + neg(v,x), the word is
-// d2 >> 5, the shift is d2 & 30 and the window test is d2 < 2 span; padding
-// (kEmpty) and ranks above the window fail it.  A 64-element group with a
+// LDS bitmap B over the rank window [lo, lo + span) of the staged list, two
+// bits per rank, one-hot by sign: rank lo + r with sign s sets bit 2r + s.
+// The sorted array L[0, nlow) holds the entries below lo.  With d2 = y - 2 lo
+// = 2 (rank - lo) + nb, the word is d2 >> 5, bit d2 & 31 is the same-sign bit
+// and bit (d2 ^ 1) & 31 the opposite-sign bit (one bit-field extract each),
+// and the window test is d2 < 2 span; padding (kEmpty) and ranks above the
+// window fail it.  A 64-element group with a
 // target below lo (a prefix of the sorted N+(v), so rare) also searches L for
 // those lanes.  A target below lo wraps d2 to at least 2^32 - 2 lo > 2^21 >=
 // 2 span, and kEmpty - 2 lo exceeds 2 span too, because the API admits at most
-// 2^31 - 2^20 vertices (ranks).
+// 2^31 - 2^20 vertices (ranks).  LOW = false is the probe of a list with no
+// entries below the window (nlow = 0), which skips the low test entirely.
+template <bool LOW>
 struct BitmapProbe {
   const uint32_t *B;
   uint32_t lo2, span2;
   const uint32_t *L;
   uint32_t nlow;
-  __device__ __forceinline__ uint32_t operator()(uint32_t y) const {
+  __device__ __forceinline__ void operator()(uint32_t y, bool &same, bool &diff) const {
     const uint32_t d2 = y - lo2;
     uint32_t w = 0;
     if (d2 < span2) w = B[d2 >> 5];
-    uint32_t q = (w >> (d2 & 30u)) & 3u;
-    if (nlow && __ballot(y < lo2)) {  // wave-uniform; no low entries: no hit below
-      if (y < lo2) q = low_lookup(L, nlow, y >> 1);
+    // v_bfe_u32 reads bits [4:0] of its offset operand: d2 & 31 needs no mask
+    same = __builtin_amdgcn_ubfe(w, d2, 1u) != 0u;
+    diff = __builtin_amdgcn_ubfe(w, d2 ^ 1u, 1u) != 0u;
+    if constexpr (LOW) {
+      const bool below = y - 1u < lo2 - 1u;  // 0 < y < lo2: 0 is no entry
+      if (__ballot(below)) {                  // wave-uniform
+        if (below) {
+          const uint32_t x = low_lookup(L, nlow, y >> 1);
+          found_as(x != kEmpty, x, y, same, diff);
+        }
+      }
     }
-    return q;
   }
 };
 
@@ -896,22 +918,22 @@ struct BitmapProbe {
 struct HashProbe {
   const uint32_t *T;
   uint32_t mask;
-  __device__ __forceinline__ uint32_t operator()(uint32_t y) const {
-    uint32_t q = 0;
-    if (y != kEmpty) {
+  __device__ __forceinline__ void operator()(uint32_t y, bool &same, bool &diff) const {
+    uint32_t hit = kEmpty;
+    if (y != kEmpty && y != 0u) {
       const uint32_t yr = y >> 1;
       uint32_t h = yr & mask;
       while (true) {
         const uint32_t x = T[h];
         if (x == kEmpty) break;
         if ((x >> 1) == yr) {
-          q = 1u | ((x & 1u) << 1);
+          hit = x;
           break;
         }
         h = (h + 1) & mask;
       }
     }
-    return q;
+    found_as(hit != kEmpty, hit, y, same, diff);
   }
 };
 
@@ -963,8 +985,9 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       if (c.pos + u * 64u >= c.eb) break;  // wave-uniform
-      const uint32_t yu = c.pos + u * 64u + lane < c.eb ? y[u] : kEmpty;
-      tally<CLS>(probe(yu), yu & 1u, e0, e1, e2);
+      bool same, diff;
+      probe(y[u], same, diff);  // lanes past the list end read 0, which misses
+      tally<CLS>(same, diff, y[u] & 1u, e0, e1, e2);
     }
     // Retire the buffer's loads here (a no-op use), so a group skipped past
     // the list end leaves nothing pending into the loop's next turn.
@@ -1115,9 +1138,13 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
       for (int u = 0; u < kUnroll; u++) {
         if (B + u * 64u >= total) break;  // wave-uniform
         const uint32_t yu = B + u * 64u + lane < total ? y[u] : kEmpty;
-        const uint32_t q = probe(yu);
-        const uint32_t neg = (uint32_t)((negm >> (li[u] & 63u)) & 1ull) + (q >> 1) + (yu & 1u);
-        acc.add((q & 1u) != 0u, neg);
+        bool same, diff;
+        probe(yu, same, diff);
+        // negatives among the triangle's three edges: the list's edge, plus
+        // 1 (diff) or 2 nb (same) for the two edges to the found vertex
+        const uint32_t neg = (uint32_t)((negm >> (li[u] & 63u)) & 1ull) +
+                             (diff ? 1u : 2u * (yu & 1u));
+        acc.add(same || diff, neg);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(y[u]));
@@ -1265,8 +1292,16 @@ __device__ __forceinline__ uint32_t window_base(uint32_t lo, uint32_t hi, uint32
 // low array L[0, nlow).
 struct Staged {
   uint32_t la, ea, d, lo, hi, span, base, nlow;
-  __device__ __forceinline__ BitmapProbe probe(const uint32_t *B, const uint32_t *L) const {
-    return BitmapProbe{B, base << 1, (hi - base + 1u) << 1, L, nlow};
+  // f(probe) with the probe of this staging; nlow is wave-uniform, so lists
+  // without low entries run a probe without the low test.
+  template <class F>
+  __device__ __forceinline__ void with_probe(const uint32_t *B, const uint32_t *L,
+                                             const F &f) const {
+    const uint32_t lo2 = base << 1, span2 = (hi - base + 1u) << 1;
+    if (nlow)
+      f(BitmapProbe<true>{B, lo2, span2, L, nlow});
+    else
+      f(BitmapProbe<false>{B, lo2, span2, L, 0u});
   }
 };
 
@@ -1289,7 +1324,7 @@ __device__ __forceinline__ Staged stage_block(const DevCsr &g, uint32_t x, uint3
     const uint32_t r = w >> 1;
     if (r >= st.base) {
       const uint32_t o = r - st.base;
-      atomicOr(&B[o >> 4], (1u | ((w & 1u) << 1)) << ((o & 15u) * 2u));
+      atomicOr(&B[o >> 4], 1u << ((o & 15u) * 2u + (w & 1u)));
       if (k == 0) *s_nlow = 0u;
     } else {
       L[k] = w;
@@ -1340,7 +1375,7 @@ __device__ __forceinline__ StagedWave stage_wave(const DevCsr &g, uint32_t x, ui
     sw.o = (w >> 1) - st.base;  // wraps for entries below the window
   }
   sw.in_w = mine && (w >> 1) >= st.base;
-  if (sw.in_w) atomicOr(&B[sw.o >> 4], (1u | ((w & 1u) << 1)) << ((sw.o & 15u) * 2u));
+  if (sw.in_w) atomicOr(&B[sw.o >> 4], 1u << ((sw.o & 15u) * 2u + (w & 1u)));
   else if (mine) L[lane] = w;  // the low entries are the lanes [0, nlow)
   st.nlow = (uint32_t)__popcll(__ballot(mine && !sw.in_w));
   wave_lds_sync();
@@ -1364,12 +1399,14 @@ __device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a,
   const uint32_t nwaves = blockDim.x >> 6;
   [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
   const Staged st = stage_block(g, a, B, L, s_nlow, 1u << tune.bitmap_span_log2);
-  stream_edges<CLS, false>(
-      g, st.la + wave, st.ea, nwaves, PushMeta{g},
-      [&](uint32_t j, uint32_t pb, uint32_t eb) {
-        return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
-      },
-      st.probe(B, L), acc);
+  st.with_probe(B, L, [&](const auto &probe) {
+    stream_edges<CLS, false>(
+        g, st.la + wave, st.ea, nwaves, PushMeta{g},
+        [&](uint32_t j, uint32_t pb, uint32_t eb) {
+          return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
+        },
+        probe, acc);
+  });
   pv_credit<CLS>(g, a, acc, before);
   unstage_block(g, st, B);
 }
@@ -1383,12 +1420,14 @@ __device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, 
   [[maybe_unused]] const uint64_t before[4] = {acc.c[0], acc.c[1], acc.c[2], acc.c[3]};
   const StagedWave sw = stage_wave(g, a, B, L, window);
   const Staged &st = sw.st;
-  stream_edges<CLS, false>(
-      g, st.la, st.ea, 1u, PushMeta{g},
-      [&](uint32_t j, uint32_t pb, uint32_t eb) {
-        return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
-      },
-      st.probe(B, L), acc);
+  st.with_probe(B, L, [&](const auto &probe) {
+    stream_edges<CLS, false>(
+        g, st.la, st.ea, 1u, PushMeta{g},
+        [&](uint32_t j, uint32_t pb, uint32_t eb) {
+          return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
+        },
+        probe, acc);
+  });
   pv_credit<CLS>(g, a, acc, before);
   unstage_wave(sw, B);
 }
@@ -1440,8 +1479,9 @@ __device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, 
   const uint32_t nwaves = blockDim.x >> 6;
   const PullChunk c = pull_chunk(g, j0, kPullBlockChunk);
   const Staged st = stage_block(g, c.v, B, L, s_nlow, 1u << tune.bitmap_span_log2);
-  pull_stream<CLS, false>(g, c.t0 + wave, c.t1, nwaves, st.d, table, st.probe(B, L), nullptr,
-                          acc);
+  st.with_probe(B, L, [&](const auto &probe) {
+    pull_stream<CLS, false>(g, c.t0 + wave, c.t1, nwaves, st.d, table, probe, nullptr, acc);
+  });
   unstage_block(g, st, B);
 }
 
@@ -1452,7 +1492,9 @@ __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, u
   const PullChunk c = pull_chunk(g, j0, kPullWaveChunk);
   const StagedWave sw = stage_wave(g, c.v, B, L, window);
   const Staged &st = sw.st;
-  pull_stream<CLS, true>(g, c.t0, c.t1, 1u, st.d, table, st.probe(B, L), H, acc);
+  st.with_probe(B, L, [&](const auto &probe) {
+    pull_stream<CLS, true>(g, c.t0, c.t1, 1u, st.d, table, probe, H, acc);
+  });
   unstage_wave(sw, B);
 }
 
