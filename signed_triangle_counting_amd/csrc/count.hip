@@ -167,49 +167,63 @@ __device__ __forceinline__ bool bitmap_fits(const TuneParams &t, uint32_t d, con
   return d - 1u <= kLowCap || span() <= (1u << t.bitmap_span_log2);
 }
 
-// A target list fits the pull kernels when it has at most kLowCap entries:
-// all of its entries below any window then fit the low array.  The bucket's
-// target class clog2(db) decides it (db <= 1024 <=> class <= 10), so binning
-// reads no list ends, and every in-edge of one target gets the same answer.
+// A target list fits the pull kernels under the rule of the bitmap kernels:
+// at most kLowCap entries (all of its entries below any window then fit the
+// low array), or at most kBitmapMaxDeg entries over a rank span within the
+// workgroup window (no entry below it).  The bucket's target class clog2(db)
+// decides the first case (db <= 1024 <=> class <= 10) without reading the
+// list; span_b() (the target list's rank span) is read only for class 11.
+// Every in-edge of one target gets the same answer.
 constexpr int kPullMaxClass = 10;  // clog2(kLowCap)
 static_assert((1u << kPullMaxClass) == kLowCap, "pull fit class");
+static_assert((2u << kPullMaxClass) == kBitmapMaxDeg, "pull fit by span: one class above");
 
-__device__ __forceinline__ bool pull_fits(const TuneParams &t, int bk) {
-  return t.allow_pull && bk % kBucketSide <= kPullMaxClass;
+template <class SpanB>
+__device__ __forceinline__ bool pull_fits(const TuneParams &t, int bk, const SpanB &span_b) {
+  if (!t.allow_pull) return false;
+  const int cls = bk % kBucketSide;
+  if (cls <= kPullMaxClass) return true;
+  return cls == kPullMaxClass + 1 && span_b() <= (1u << t.bitmap_span_log2);
 }
 
 // Final strategy of an edge in bucket bk: the bin's choice, demoted when the
-// list it stages does not fit the LDS structure -- pull (target list) to
-// bitmap, bitmap (source list of dfull_a entries over span_a ranks) to hash,
-// hash to merge.  Binning (from the stored bucket) and the vertex-grouped
-// kernels (through edge_strategy) apply the same rule, so both agree edge by
-// edge.  span_a is read only when the chain reaches the bitmap test (span_of
-// below).
-template <class Span>
+// list it stages does not fit the LDS structure -- pull (target list, rank
+// span span_b) to bitmap, bitmap (source list of dfull_a entries over span_a
+// ranks) to hash, hash to merge.  Binning (from the stored bucket) and the
+// vertex-grouped kernels (through edge_strategy) apply the same rule, so both
+// agree edge by edge.  The spans are read only when the chain reaches their
+// test (span_of below).
+template <class SpanA, class SpanB>
 __device__ __forceinline__ uint8_t bucket_strategy(const uint8_t *table, const TuneParams &t,
-                                                   uint32_t dfull_a, const Span &span_a, int bk) {
+                                                   uint32_t dfull_a, const SpanA &span_a,
+                                                   const SpanB &span_b, int bk) {
   uint8_t s = table[bk];
-  if (s == S_PULL && !pull_fits(t, bk)) s = S_BITMAP;
+  if (s == S_PULL && !pull_fits(t, bk, span_b)) s = S_BITMAP;
   if (s == S_BITMAP && !bitmap_fits(t, dfull_a, span_a)) s = S_HASH;
   if (s == S_HASH && dfull_a > (1u << t.hash_max_log2)) s = merge_variant(bk, t);
   return s;
 }
 
-__device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table,
-                                                 const TuneParams &t,
-                                                 uint32_t dfull_a, uint32_t span_a,
-                                                 uint32_t da_eff, uint32_t db) {
-  if (da_eff == 0 || db == 0) return S_SKIP;
-  return bucket_strategy(table, t, dfull_a, [&] { return span_a; }, bucket_of(da_eff, db));
-}
-
-// Bucket byte of an edge that cannot close a triangle (da = 0 or db = 0).
-constexpr uint8_t kNoBucket = 0xFF;
-
 __device__ __forceinline__ uint32_t span_of(const uint32_t *__restrict__ col,
                                             uint32_t la, uint32_t ea) {
   return ea > la ? (col[ea - 1] >> 1) - (col[la] >> 1) + 1u : 0u;
 }
+
+// Strategy of edge (a -> b) seen from a vertex-grouped kernel: the source
+// list has dfull_a entries over span_a ranks, da_eff of them after b; b's
+// list is col[pb, eb).
+__device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table, const TuneParams &t,
+                                                 uint32_t dfull_a, uint32_t span_a,
+                                                 uint32_t da_eff, const uint32_t *col,
+                                                 uint32_t pb, uint32_t eb) {
+  const uint32_t db = eb - pb;
+  if (da_eff == 0 || db == 0) return S_SKIP;
+  return bucket_strategy(table, t, dfull_a, [&] { return span_a; },
+                         [&] { return span_of(col, pb, eb); }, bucket_of(da_eff, db));
+}
+
+// Bucket byte of an edge that cannot close a triangle (da = 0 or db = 0).
+constexpr uint8_t kNoBucket = 0xFF;
 
 // Rank of this lane among lanes of the wave with the same 8-bit value.
 __device__ __forceinline__ void wave_match8(uint32_t d, bool valid, uint32_t &rank,
@@ -364,8 +378,13 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
       const uint32_t bk = bk8[e];
       if (bk != kNoBucket) {
         const uint32_t la = g.off[a], ea = g.off[a + 1];
-        s = bucket_strategy(table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
-                            (int)bk);
+        s = bucket_strategy(
+            table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
+            [&] {
+              const uint32_t b = g.col[e] >> 1;
+              return span_of(g.col, g.off[b], g.off[b + 1]);
+            },
+            (int)bk);
         if (s == S_PULL) v = g.col[e] >> 1;
       }
     }
@@ -578,8 +597,9 @@ __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
       const uint32_t a = g.src[e], b = g.col[e] >> 1;
       const uint32_t la = g.off[a], ea = g.off[a + 1];
       const uint32_t db = g.off[b + 1] - g.off[b];
-      const uint8_t s = bucket_strategy(table, tune, ea - la,
-                                        [&] { return span_of(g.col, la, ea); }, (int)bk);
+      const uint8_t s = bucket_strategy(
+          table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
+          [&] { return span_of(g.col, g.off[b], g.off[b + 1]); }, (int)bk);
       const bool small = ea - la <= kWaveMaxDeg, small_b = db <= kWaveMaxDeg;
       slot = s == S_SKIP         ? 8u  // left out by a partial strategy table
              : s == S_MERGE_LANE ? 0u
@@ -1238,7 +1258,7 @@ __device__ __forceinline__ void hash_vertex_block(const DevCsr &g, uint32_t a, u
   stream_edges<CLS, false>(
       g, la + wave, ea, nwaves, PushMeta{g},
       [&](uint32_t j, uint32_t pb, uint32_t eb) {
-        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_HASH;
+        return edge_strategy(table, tune, d, span, ea - j - 1, g.col, pb, eb) == S_HASH;
       },
       HashProbe{T, mask}, acc);
   pv_credit<CLS>(g, a, acc, before);
@@ -1272,7 +1292,7 @@ __device__ __forceinline__ void hash_vertex_wave(const DevCsr &g, uint32_t a, ui
   stream_edges<CLS, false>(
       g, la, ea, 1u, PushMeta{g},
       [&](uint32_t j, uint32_t pb, uint32_t eb) {
-        return edge_strategy(table, tune, d, span, ea - j - 1, eb - pb) == S_HASH;
+        return edge_strategy(table, tune, d, span, ea - j - 1, g.col, pb, eb) == S_HASH;
       },
       HashProbe{T, mask}, acc);
   pv_credit<CLS>(g, a, acc, before);
@@ -1403,7 +1423,8 @@ __device__ __forceinline__ void bitmap_vertex_block(const DevCsr &g, uint32_t a,
     stream_edges<CLS, false>(
         g, st.la + wave, st.ea, nwaves, PushMeta{g},
         [&](uint32_t j, uint32_t pb, uint32_t eb) {
-          return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
+          return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, g.col, pb, eb) ==
+                 S_BITMAP;
         },
         probe, acc);
   });
@@ -1424,7 +1445,8 @@ __device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, 
     stream_edges<CLS, false>(
         g, st.la, st.ea, 1u, PushMeta{g},
         [&](uint32_t j, uint32_t pb, uint32_t eb) {
-          return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, eb - pb) == S_BITMAP;
+          return edge_strategy(table, tune, st.d, st.span, st.ea - j - 1, g.col, pb, eb) ==
+                 S_BITMAP;
         },
         probe, acc);
   });
