@@ -17,7 +17,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HIP_SOURCES = ["csr_build.hip", "count.hip", "mlp.hip"]
-HIP_HOST_SOURCES = ["api.cpp"]          # host code that calls the HIP runtime
+HIP_HOST_SOURCES = ["api.cpp", "upload.cpp"]  # host code that calls the HIP runtime
 HOST_SOURCES = ["gen.cpp", "graph_io.cpp"]
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -45,6 +46,7 @@ struct stc_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t marks[kMarks] = {};
   DevBuf batch_counts, batch_vbase;
+  UploadRing ring;  // pinned staging for host edge lists (allocated on first use)
 };
 
 namespace {
@@ -312,6 +314,7 @@ void stc_destroy(stc_ctx *c) {
   for (auto &ev : c->marks)
     if (ev) (void)hipEventDestroy(ev);
   if (c->dp) (void)hipFree(c->dp);
+  c->ring.release();
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -336,16 +339,24 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
         (e = c->bb.in_v.ensure(mm * 4)) != hipSuccess ||
         (e = c->bb.in_s.ensure(mm)) != hipSuccess)
       return hip_fail(c, e, "upload alloc");
-    e = hipEventRecord(c->ev[0], c->st);
-    if (g->m && e == hipSuccess)
-      e = hipMemcpyAsync(c->bb.in_u.p, g->u, g->m * 4, hipMemcpyHostToDevice, c->st);
-    if (g->m && e == hipSuccess)
-      e = hipMemcpyAsync(c->bb.in_v.p, g->v, g->m * 4, hipMemcpyHostToDevice, c->st);
-    if (g->m && e == hipSuccess)
-      e = hipMemcpyAsync(c->bb.in_s.p, g->sign, g->m, hipMemcpyHostToDevice, c->st);
-    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
-    if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms_up, c->ev[0], c->ev[1]);
+    if (g->m >= kStageMinLines && (e = c->ring.ensure()) != hipSuccess)
+      return hip_fail(c, e, "staging alloc");
+    e = hipStreamSynchronize(c->st);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (g->m >= kStageMinLines) {
+      if (e == hipSuccess)
+        e = c->ring.upload({EdgePiece{g->u, g->v, g->sign, g->m}}, c->bb.in_u.as<uint32_t>(),
+                           c->bb.in_v.as<uint32_t>(), c->bb.in_s.as<int8_t>(), c->st);
+    } else if (g->m) {
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(c->bb.in_u.p, g->u, g->m * 4, hipMemcpyHostToDevice, c->st);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(c->bb.in_v.p, g->v, g->m * 4, hipMemcpyHostToDevice, c->st);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(c->bb.in_s.p, g->sign, g->m, hipMemcpyHostToDevice, c->st);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    ms_up = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (e != hipSuccess) return hip_fail(c, e, "upload");
     du = c->bb.in_u.as<uint32_t>();
     dv = c->bb.in_v.as<uint32_t>();
@@ -552,11 +563,21 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
   uint32_t *d_vbase = (uint32_t *)(d_ebase + (k + 1));
   uint32_t *d_bad = d_vbase + (k + 1);
   float ms_up = 0.f;
-  e = hipEventRecord(c->ev[0], c->st);
+  if (m >= kStageMinLines && (e = c->ring.ensure()) != hipSuccess)
+    return hip_fail(c, e, "staging alloc");
+  e = hipStreamSynchronize(c->st);
+  const auto t0 = std::chrono::steady_clock::now();
   // find the first graph with lines for the contiguous fast path
   uint32_t first = 0;
   while (first < k && graphs[first].m == 0) first++;
-  if (e == hipSuccess && m && contiguous && first < k) {
+  if (e == hipSuccess && m >= kStageMinLines) {
+    // the graphs back to back through the pinned staging buffers
+    std::vector<EdgePiece> pieces;
+    pieces.reserve(k);
+    for (uint32_t i = 0; i < k; i++)
+      if (graphs[i].m) pieces.push_back(EdgePiece{graphs[i].u, graphs[i].v, graphs[i].sign, graphs[i].m});
+    e = c->ring.upload(pieces, du, dv, ds, c->st);
+  } else if (e == hipSuccess && m && contiguous && first < k) {
     e = hipMemcpyAsync(du, graphs[first].u, m * 4, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(dv, graphs[first].v, m * 4, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(ds, graphs[first].sign, m, hipMemcpyHostToDevice, c->st);
@@ -573,9 +594,8 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
     e = hipMemcpyAsync(d_ebase, ebase.data(), (k + 1) * 8, hipMemcpyHostToDevice, c->st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(d_vbase, vbase.data(), (k + 1) * 4, hipMemcpyHostToDevice, c->st);
-  if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
-  if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
-  if (e == hipSuccess) e = hipEventElapsedTime(&ms_up, c->ev[0], c->ev[1]);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  ms_up = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (e != hipSuccess) return hip_fail(c, e, "batch upload");
   // shift ids, build the union CSR
   e = hipEventRecord(c->ev[0], c->st);

@@ -103,6 +103,30 @@ struct DevCsr {
   unsigned long long *pv = nullptr;
 };
 
+// ---- host -> device upload through pinned staging ----------------------------
+// One graph's edge arrays (host memory, m lines).
+struct EdgePiece {
+  const uint32_t *u, *v;
+  const int8_t *s;
+  uint64_t m;
+};
+
+// Two pinned staging buffers of kStageLines lines each (u, then v, then the
+// sign bytes) and their copy-done events, kept by a context after first use.
+constexpr uint64_t kStageLines = 1ull << 23;     // 72 MiB per buffer
+constexpr uint64_t kStageMinLines = 1ull << 20;  // smaller inputs are copied directly
+struct UploadRing {
+  void *host[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  ~UploadRing() { release(); }
+  hipError_t ensure();
+  void release();
+  // Copy the pieces back to back into du / dv / ds (device) on stream st and
+  // wait for the copies to finish.
+  hipError_t upload(const std::vector<EdgePiece> &pieces, uint32_t *du, uint32_t *dv, int8_t *ds,
+                    hipStream_t st);
+};
+
 // ---- device memory --------------------------------------------------------
 struct DevBuf {
   void *p = nullptr;

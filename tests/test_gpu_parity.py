@@ -287,6 +287,30 @@ def test_partial_tables_strategy_invariant(ctx, scale, ef):
     assert total == oracle_counts(g.n, g.u, g.v, g.sign)
 
 
+def test_staged_upload_partial_chunk(ctx):
+    """Host edge lists of 2^20 lines and more go through the pinned staging
+    buffers in chunks of 2^23 lines; a graph one partial chunk past a chunk
+    boundary, and a batch whose graphs straddle chunk boundaries, count as
+    the oracle says."""
+    rng = np.random.default_rng(77)
+    n = 3_000_000
+    m = (1 << 23) + 4321
+    u, v, s = random_graph(rng, n, m, p_neg=0.3)
+    g = stc.Graph(u=u, v=v, sign=s, n=n)
+    say("staged upload", m)
+    assert ctx.count(g) == oracle_counts(n, u, v, s)
+    assert ctx.stats()["m_in"] == m
+    say("staged batch upload")
+    graphs = []
+    for i in range(9):
+        mi = int(rng.integers(900_000, 1_100_000))
+        ui, vi, si = random_graph(rng, 200_000, mi, p_neg=0.4)
+        graphs.append(stc.Graph(u=ui, v=vi, sign=si, n=200_000))
+    got = ctx.count_batch(graphs)
+    for gi, c in zip(graphs, got):
+        assert tuple(c) == oracle_counts(gi.n, gi.u, gi.v, gi.sign)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_mixed_strategy_tables(ctx, seed):
     """A table mixing merge, hash, bitmap and pull bucket by bucket counts
