@@ -641,7 +641,7 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
                             stc_counts *total, CountResult *res_out) {
   const uint32_t k = c->batch_k;
   c->hp.mode = mode;
-  c->hp.tune.allow_pull = 0;  // the batched kernel has no pull items
+  c->hp.tune.allow_pull = 1;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
@@ -725,7 +725,7 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
     return fail(c, STC_E_ARG, "bad dispatch");
   if (opts && opts->classes) return fail(c, STC_E_ARG, "batched counting reports two classes");
   c->hp.mode = dispatch;
-  c->hp.tune.allow_pull = 0;  // the batched kernel has no pull items
+  c->hp.tune.allow_pull = 1;
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();

@@ -1492,11 +1492,12 @@ __device__ __forceinline__ void pull_stream(const DevCsr &g, uint32_t first, uin
     stream_packed<CLS>(g, first, t1, step, PullMeta{g}, keep, probe, H, acc);
 }
 
+// Both return the chunk's target v (its graph owns the chunk's triangles).
 template <bool CLS>
-__device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, uint32_t *B,
-                                                  uint32_t *L, uint32_t *s_nlow,
-                                                  const uint8_t *table, const TuneParams &tune,
-                                                  Acc<CLS> &acc) {
+__device__ __forceinline__ uint32_t pull_vertex_block(const DevCsr &g, uint32_t j0, uint32_t *B,
+                                                      uint32_t *L, uint32_t *s_nlow,
+                                                      const uint8_t *table,
+                                                      const TuneParams &tune, Acc<CLS> &acc) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nwaves = blockDim.x >> 6;
   const PullChunk c = pull_chunk(g, j0, kPullBlockChunk);
@@ -1505,19 +1506,23 @@ __device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, 
     pull_stream<CLS, false>(g, c.t0 + wave, c.t1, nwaves, st.d, table, probe, nullptr, acc);
   });
   unstage_block(g, st, B);
+  return c.v;
 }
 
-template <bool CLS>
-__device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t *B,
-                                                 uint32_t *L, uint32_t window, uint64_t *H,
-                                                 const uint8_t *table, Acc<CLS> &acc) {
+// PACKED = false streams one list at a time (no heads array; fewer VGPRs, for
+// kernels that also run other item kinds).
+template <bool CLS, bool PACKED = true>
+__device__ __forceinline__ uint32_t pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t *B,
+                                                     uint32_t *L, uint32_t window, uint64_t *H,
+                                                     const uint8_t *table, Acc<CLS> &acc) {
   const PullChunk c = pull_chunk(g, j0, kPullWaveChunk);
   const StagedWave sw = stage_wave(g, c.v, B, L, window);
   const Staged &st = sw.st;
   st.with_probe(B, L, [&](const auto &probe) {
-    pull_stream<CLS, true>(g, c.t0, c.t1, 1u, st.d, table, probe, H, acc);
+    pull_stream<CLS, PACKED>(g, c.t0, c.t1, 1u, st.d, table, probe, H, acc);
   });
   unstage_wave(sw, B);
+  return c.v;
 }
 
 // Pop one item for the whole workgroup from an XCD queue.
@@ -1669,15 +1674,20 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 // Items, in queue order (popped from the top, so heaviest kinds first):
 //   merge-lane chunks of kLaneItem edges (one per thread), merge-wave chunks
 //   of one edge per wave, wave-hash and wave-bitmap chunks of one vertex per
-//   wave, hash vertices, bitmap vertices.  Every item's triangles are added to
-//   the counters of the graph owning its source vertex (graph_of is indexed by
-//   rank; each graph's ranks are one contiguous range).
+//   wave, wave-pull chunks of one in-edge chunk per wave, hash vertices,
+//   bitmap vertices, workgroup-pull in-edge chunks.  Every item's triangles
+//   are added to the counters of the graph owning its vertex (the source, or
+//   for pull the target: the graphs are disjoint, so its in-edges come from
+//   its own graph; graph_of is indexed by rank; each graph's ranks are one
+//   contiguous range).
 //   A workgroup works on one item at a time, so all LDS structures share one
 //   region R, kept zero between items: the workgroup bitmap R[0, bitmap_words)
 //   with its low array after it, the workgroup hash table R[0, hash_slots),
 //   the wave hash tables R[w kWaveCap, +kWaveCap) and the wave windows
-//   R[w (window_words + kWaveMaxDeg), ...) with their low arrays.  Bitmaps
-//   clear what they set; tables and low arrays are cleared after use.
+//   R[w (window_words + kWaveMaxDeg), ...) with their low arrays (wave-bitmap
+//   and wave-pull items).  Bitmaps clear what they set; tables and low arrays
+//   are cleared after use.  Pull streams one list at a time here: the packed
+//   stream's registers would halve this kernel's occupancy for every item.
 constexpr uint32_t kBatchBlock = 512;
 constexpr uint32_t kLaneItem = kBatchBlock;
 constexpr uint32_t kWavesPerBatchBlock = kBatchBlock / 64;
@@ -1705,7 +1715,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
     DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vh,
     const uint32_t *__restrict__ vb, const uint32_t *__restrict__ vw,
-    const uint32_t *__restrict__ vwb, const uint32_t *__restrict__ graph_of,
+    const uint32_t *__restrict__ vwb, const uint32_t *__restrict__ vp,
+    const uint32_t *__restrict__ vwp, const uint32_t *__restrict__ graph_of,
     unsigned long long *__restrict__ per_graph, uint32_t bitmap_words, uint32_t hash_slots,
     uint32_t window_words, uint32_t region_words) {
   extern __shared__ uint32_t R[];
@@ -1722,13 +1733,14 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
   uint32_t *const LW = BW + window_words;                         // its low array
   const uint32_t window = window_words * 16u;
   const uint32_t nl = ctrl->len[0], nwv = ctrl->len[1], nh = ctrl->len[2], nb = ctrl->len[3],
-                 nhw = ctrl->len[4], nbw = ctrl->len[5];
+                 nhw = ctrl->len[4], nbw = ctrl->len[5], np = ctrl->len[6], nwp = ctrl->len[7];
   const uint32_t i_lane = (nl + kLaneItem - 1) / kLaneItem;
   const uint32_t i_wave = (nwv + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
   const uint32_t i_hw = (nhw + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
   const uint32_t i_bw = (nbw + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
-  const uint32_t b1 = i_lane, b2 = b1 + i_wave, b3 = b2 + i_hw, b3w = b3 + i_bw, b4 = b3w + nh,
-                 total = b4 + nb;
+  const uint32_t i_wp = (nwp + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
+  const uint32_t b1 = i_lane, b2 = b1 + i_wave, b3 = b2 + i_hw, b3w = b3 + i_bw,
+                 b3p = b3w + i_wp, b4 = b3p + nh, b5 = b4 + nb, total = b5 + np;
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<false> acc;
   uint32_t it;
@@ -1768,15 +1780,30 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
         acc.flush_graph(per_graph, graph_of[a]);
         LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
       }
+    } else if (it < b3p) {  // wave-pull chunk
+      const uint32_t i = (it - b3w) * kWavesPerBatchBlock + wave;
+      if (i < nwp) {
+        const uint32_t j0 = __builtin_amdgcn_readfirstlane(vwp[i]);
+        const uint32_t v = pull_vertex_wave<false, false>(g, j0, BW, LW, window, nullptr, table,
+                                                          acc);
+        acc.flush_graph(per_graph, graph_of[v]);
+        LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
+      }
     } else if (it < b4) {  // hash vertex
-      const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b3w]);
+      const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b3p]);
       hash_vertex_block<false>(g, a, R, table, tune, acc);
       acc.flush_graph(per_graph, graph_of[a]);
       for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
-    } else {  // bitmap vertex
+    } else if (it < b5) {  // bitmap vertex
       const uint32_t a = __builtin_amdgcn_readfirstlane(vb[it - b4]);
       bitmap_vertex_block<false>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
       acc.flush_graph(per_graph, graph_of[a]);
+      for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
+    } else {  // workgroup-pull chunk
+      const uint32_t j0 = __builtin_amdgcn_readfirstlane(vp[it - b5]);
+      const uint32_t v =
+          pull_vertex_block<false>(g, j0, R, R + bitmap_words, &s_nlow, table, tune, acc);
+      acc.flush_graph(per_graph, graph_of[v]);
       for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
     }
   }
@@ -2029,7 +2056,8 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
   const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (150 * 1024) / lds));
   hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
                      cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vb, L.vw,
-                     L.vwb, graph_of, per_graph, (1u << hp.tune.bitmap_span_log2) / 16u,
+                     L.vwb, L.vp, L.vwp, graph_of, per_graph,
+                     (1u << hp.tune.bitmap_span_log2) / 16u,
                      4u << hp.tune.hash_max_log2, (1u << hp.tune.wave_window_log2) / 16u,
                      (uint32_t)batch_region_words(hp.tune));
   return hipGetLastError();

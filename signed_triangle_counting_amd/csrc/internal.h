@@ -67,8 +67,8 @@ struct TuneParams {
   int hash_block;         // threads per workgroup for hash/bitmap
   int fixed_skew_log2;    // fixed rule: skew >= 2^x -> hash
   int fixed_hub_log2;     // fixed rule: d+(v) >= 2^x (or source) -> bitmap
-  int allow_pull;         // set per call: 0 where no kernel counts pull edges
-                          // (the batched kernel), which then fall back to bitmap
+  int allow_pull;         // 1: pull buckets are counted by the pull kernels (single
+                          // graphs and batches); 0: they fall back to bitmap
 };
 
 struct DispatchParams {
