@@ -14,8 +14,12 @@
 //   merge-wave  one wave64 per edge, 64x64 tiles, ds_bpermute binary search
 //   hash        one workgroup per source u: N+(u) in an LDS hash table,
 //               waves stream N+(v) for u's hash edges and probe
-//   bitmap      one workgroup per source u: N+(u) as a 2-bit (present,
-//               negative) LDS bitmap over its rank span, waves stream N+(v)
+//   bitmap      one workgroup (or wave) per source u: N+(u) as an LDS bitmap
+//               over a rank window, two bits per rank one-hot by sign, waves
+//               stream N+(v)
+//   pull        one workgroup (or wave) per chunk of the in-edges of a target
+//               v: N+(v) as such a bitmap, waves stream the part of N+(u)
+//               after v for each in-edge (u->v)
 // A triangle {u,v,x} with x in N+(u) and N+(v) has parity
 // neg(uv) ^ neg(ux) ^ neg(vx); parity 0 is balanced.  Every kernel reduces
 // with wave64 ballot + popcount into 64-bit counters.
@@ -1306,10 +1310,10 @@ __device__ __forceinline__ uint32_t window_base(uint32_t lo, uint32_t hi, uint32
   return hi - lo + 1u > window ? hi - window + 1u : lo;
 }
 
-// A vertex x's out-list N+(x) staged in LDS: a 2-bit (present, negative)
-// bitmap over the rank window [base, hi] (entry r = bits 2r, 2r+1 for rank
-// base + r) and the entries below base (a prefix of the sorted list) in the
-// low array L[0, nlow).
+// A vertex x's out-list N+(x) staged in LDS: a bitmap over the rank window
+// [base, hi], one-hot by sign (rank base + r with sign s sets bit 2r + s),
+// and the entries below base (a prefix of the sorted list) in the low array
+// L[0, nlow).
 struct Staged {
   uint32_t la, ea, d, lo, hi, span, base, nlow;
   // f(probe) with the probe of this staging; nlow is wave-uniform, so lists
