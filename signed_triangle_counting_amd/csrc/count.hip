@@ -261,28 +261,44 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst, uint64_t v) {
 }
 
 // ---- XCD-aware work queue ---------------------------------------------------
-// `len` items are split into 8 contiguous ranges, one per XCD (workgroups are
-// dispatched round-robin over the 8 XCDs of an MI355X, so blockIdx % 8 names
-// the XCD and a range's vertices share that XCD's L2).  Each range has its own
-// counter on its own 128-byte line and is drained from its end (highest ranks,
-// the heaviest vertices, first); a drained range sends the caller on to the
-// next XCD's range.  Returns false when every range is empty.  (Kernels read
-// their list length first and exit at once when it is zero.)
+// `len` items are split over the 8 XCDs (workgroups are dispatched round-robin
+// over the 8 XCDs of an MI355X, so blockIdx % 8 names the XCD).  Each XCD's
+// share has its own counter on its own 128-byte line and is drained from the
+// top (highest ranks, the heaviest vertices, first); a drained share sends the
+// caller on to the next XCD's.  Returns false when every share is empty.
+// (Kernels read their list length first and exit at once when it is zero.)
+//   Contiguous shares (workgroup kernels: one vertex per pop) keep a range of
+// vertices on one XCD's L2.  INTERLEAVE deals pops of `chunk` items out
+// round-robin instead (pop c of the list to XCD c % 8; wave kernels), so the
+// heavy top of a list is spread over all XCDs: it took the wave-pull kernel
+// from 0.70 to 0.59 ms on RMAT-22.
 struct QueueCursor {
   uint32_t q, tried;
 };
 
+template <bool INTERLEAVE = false>
 __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uint32_t chunk,
                                           QueueCursor &cur, uint32_t &first,
                                           uint32_t &count) {
   while (cur.tried < 8) {
-    const uint32_t lo = (uint32_t)(((uint64_t)len * cur.q) >> 3);
-    const uint32_t hi = (uint32_t)(((uint64_t)len * (cur.q + 1)) >> 3);
-    const uint32_t k = atomicAdd(&ctr[cur.q][0], chunk);
-    if (k < hi - lo) {
-      count = min(chunk, hi - lo - k);
-      first = hi - k - count;  // items [first, first + count), taken from the top
-      return true;
+    if constexpr (INTERLEAVE) {
+      const uint32_t npops = (len + chunk - 1) / chunk;
+      const uint32_t mine = npops > cur.q ? (npops - cur.q + 7) / 8 : 0u;  // pops q, q+8, ...
+      const uint32_t k = atomicAdd(&ctr[cur.q][0], 1u);
+      if (k < mine) {
+        first = (npops - 1u - (cur.q + 8u * k)) * chunk;  // from the top
+        count = min(chunk, len - first);
+        return true;
+      }
+    } else {
+      const uint32_t lo = (uint32_t)(((uint64_t)len * cur.q) >> 3);
+      const uint32_t hi = (uint32_t)(((uint64_t)len * (cur.q + 1)) >> 3);
+      const uint32_t k = atomicAdd(&ctr[cur.q][0], chunk);
+      if (k < hi - lo) {
+        count = min(chunk, hi - lo - k);
+        first = hi - k - count;  // items [first, first + count), taken from the top
+        return true;
+      }
     }
     cur.q = (cur.q + 1) & 7u;
     cur.tried++;
@@ -1588,7 +1604,8 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   Acc<CLS> acc;
   while (true) {
     uint32_t first = 0, count = 0;
-    if (lane == 0 && !queue_pop(ctrl->queue[2], len, kWaveChunk, cur, first, count)) count = 0;
+    if (lane == 0 && !queue_pop<true>(ctrl->queue[2], len, kWaveChunk, cur, first, count))
+      count = 0;
     first = __shfl(first, 0, 64);
     count = __shfl(count, 0, 64);
     if (count == 0) break;
@@ -1658,7 +1675,8 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   Acc<CLS> acc;
   while (true) {
     uint32_t first = 0, count = 0;
-    if (lane == 0 && !queue_pop(ctrl->queue[PULL ? 6 : 4], len, kWaveChunk, cur, first, count))
+    if (lane == 0 &&
+        !queue_pop<true>(ctrl->queue[PULL ? 6 : 4], len, kWaveChunk, cur, first, count))
       count = 0;
     first = __shfl(first, 0, 64);
     count = __shfl(count, 0, 64);
