@@ -324,11 +324,11 @@ __global__ __launch_bounds__(256) void k_bin_hist(DevCsr g, Ctrl *ctrl,
     bool valid = false;
     uint32_t bk = 0;
     if (in) {
-      uint32_t a = g.src[e], b = g.col[e] >> 1;
-      uint32_t da = g.off[a + 1] - (uint32_t)e - 1u;
-      uint32_t db = g.off[b + 1] - g.off[b];
-      valid = da && db;
-      if (valid) bk = (uint32_t)bucket_of(da, db);
+      const uint32_t a = g.src[e], b = g.col[e] >> 1;
+      const uint32_t da = g.off[a + 1] - (uint32_t)e - 1u;
+      const uint32_t cb = g.dcls[b];  // the target's class: one byte per edge gathered
+      valid = da && cb != 0xFFu;
+      if (valid) bk = (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb;
       bk8[e] = valid ? (uint8_t)bk : kNoBucket;
     }
     uint32_t rank, cnt;

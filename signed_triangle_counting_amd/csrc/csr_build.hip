@@ -327,6 +327,17 @@ __global__ void k_split_csr(const uint64_t *__restrict__ key2, uint64_t m,
   }
 }
 
+// Out-degree class of every rank: min(ceil_log2 d+(v), 15), 0xFF for an
+// empty list (the binning pass's target side of an edge's bucket).
+__global__ void k_dcls(const uint32_t *__restrict__ off, uint32_t n, uint8_t *__restrict__ dcls) {
+  for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = off[v + 1] - off[v];
+    const uint32_t c = d <= 1 ? 0u : 32u - (uint32_t)__clz(d - 1u);
+    dcls[v] = d ? (uint8_t)min(c, 15u) : (uint8_t)0xFF;
+  }
+}
+
 // Keys of the transposed index: (target rank, edge id); sorted on the target
 // bits (stably, so edge ids stay ascending within a target).
 __global__ void k_tkeys(const uint32_t *__restrict__ col, uint64_t m, uint64_t *__restrict__ key) {
@@ -625,6 +636,11 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
                      bb.col.as<uint32_t>(), bb.src.as<uint32_t>(), fdeg);
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(fdeg, bb.off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
+  // out-degree class per rank (binning reads one byte per edge's target)
+  STC_TRY(bb.dcls.ensure((uint64_t)n + 1));
+  hipLaunchKernelGGL(k_dcls, dim3(grid_for((uint64_t)n + 1, T)), dim3(T), 0, st,
+                     bb.off.as<uint32_t>(), n, bb.dcls.as<uint8_t>());
+  STC_TRY(hipGetLastError());
   // transposed index for the pull strategy: in-edge records grouped by target rank
   STC_TRY(bb.tin.ensure((m ? m : 1) * sizeof(InEdge)));
   STC_TRY(bb.tin_off.ensure(((uint64_t)n + 1) * 4));
