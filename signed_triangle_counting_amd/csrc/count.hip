@@ -1136,10 +1136,9 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
     if (total == 0u) continue;
     const uint32_t D = pb - P;  // modulo 2^32: f + D = pb + (f - P)
     const uint64_t negm = __ballot(kept && (w & 1u));
-    // list index of each slot of the unit at B, and its loads (unconditional:
-    // slots past the end read col[0] and are masked when probed)
-    auto prepare = [&](uint32_t B, uint32_t(&y)[kUnroll], uint32_t(&li)[kUnroll],
-                       bool only_used) {
+    // list index of each slot of the unit at B, and its loads (rounds past
+    // the end load nothing)
+    auto prepare = [&](uint32_t B, uint32_t(&y)[kUnroll], uint32_t(&li)[kUnroll]) {
       wave_lds_sync();  // the previous unit's reads of H are done
       H[lane] = 0ull;
       wave_lds_sync();
@@ -1166,52 +1165,34 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
       wave_lds_sync();
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
-        if (only_used && B + u * 64u >= total) break;  // wave-uniform
+        if (B + u * 64u >= total) break;  // wave-uniform
         const uint32_t f = B + u * 64u + lane;
         li[u] = (uint32_t)Hb[u * 64u + lane] - 1u;
         const uint32_t d = (uint32_t)__shfl((int)D, (int)(li[u] & 63u), 64);
-        y[u] = col[f < total ? f + d : 0u];
+        y[u] = f < total ? col[f + d] : kEmpty;
       }
     };
     auto consume = [&](uint32_t B, const uint32_t(&y)[kUnroll], const uint32_t(&li)[kUnroll]) {
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (B + u * 64u >= total) break;  // wave-uniform
-        const uint32_t yu = B + u * 64u + lane < total ? y[u] : kEmpty;
         bool same, diff;
-        probe(yu, same, diff);
+        probe(y[u], same, diff);
         // negatives among the triangle's three edges: the list's edge, plus
         // 1 (diff) or 2 nb (same) for the two edges to the found vertex
         const uint32_t neg = (uint32_t)((negm >> (li[u] & 63u)) & 1ull) +
-                             (diff ? 1u : 2u * (yu & 1u));
+                             (diff ? 1u : 2u * (y[u] & 1u));
         acc.add(same || diff, neg);
       }
-#pragma unroll
-      for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(y[u]));
     };
-    // two units in turn: one unit's loads are in flight while the other is
-    // probed.  The next unit is prepared unconditionally (past the end all
-    // its slots read col[0]), so every wait counts exactly the loads it needs.
-    uint32_t ya[kUnroll], la[kUnroll], yb[kUnroll], lb[kUnroll];
-    uint32_t B = 0;
-    if (total <= kPackUnit) {  // one unit: nothing to overlap, load only its used rounds
-      prepare(B, ya, la, true);
-      consume(B, ya, la);
-      continue;
+    // One unit at a time: a unit already holds up to kPackUnit entries of many
+    // lists, and a second buffer in flight measured no faster while it cost
+    // the registers that let the batched kernel run packed pull items.
+    uint32_t y[kUnroll], li[kUnroll];
+    for (uint32_t B = 0; B < total; B += kPackUnit) {
+      prepare(B, y, li);
+      consume(B, y, li);
     }
-    prepare(B, ya, la, false);
-    while (true) {
-      prepare(B + kPackUnit, yb, lb, false);
-      consume(B, ya, la);
-      B += kPackUnit;
-      if (B >= total) break;
-      prepare(B + kPackUnit, ya, la, false);
-      consume(B, yb, lb);
-      B += kPackUnit;
-      if (B >= total) break;
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) asm volatile("" ::"v"(ya[u]), "v"(yb[u]));
   }
 }
 
@@ -1529,9 +1510,7 @@ __device__ __forceinline__ uint32_t pull_vertex_block(const DevCsr &g, uint32_t 
   return c.v;
 }
 
-// PACKED = false streams one list at a time (no heads array; fewer VGPRs, for
-// kernels that also run other item kinds).
-template <bool CLS, bool PACKED = true>
+template <bool CLS>
 __device__ __forceinline__ uint32_t pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t *B,
                                                      uint32_t *L, uint32_t window, uint64_t *H,
                                                      const uint8_t *table, Acc<CLS> &acc) {
@@ -1539,7 +1518,7 @@ __device__ __forceinline__ uint32_t pull_vertex_wave(const DevCsr &g, uint32_t j
   const StagedWave sw = stage_wave(g, c.v, B, L, window);
   const Staged &st = sw.st;
   st.with_probe(B, L, [&](const auto &probe) {
-    pull_stream<CLS, PACKED>(g, c.t0, c.t1, 1u, st.d, table, probe, H, acc);
+    pull_stream<CLS, true>(g, c.t0, c.t1, 1u, st.d, table, probe, H, acc);
   });
   unstage_wave(sw, B);
   return c.v;
@@ -1693,23 +1672,30 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 }
 
 // ---- batched counting: one persistent kernel over the union of k graphs ----
-// Items, in queue order (popped from the top, so heaviest kinds first):
-//   merge-lane chunks of kLaneItem edges (one per thread), merge-wave chunks
-//   of one edge per wave, wave-hash and wave-bitmap chunks of one vertex per
-//   wave, wave-pull chunks of one in-edge chunk per wave, hash vertices,
-//   bitmap vertices, workgroup-pull in-edge chunks.  Every item's triangles
-//   are added to the counters of the graph owning its vertex (the source, or
-//   for pull the target: the graphs are disjoint, so its in-edges come from
-//   its own graph; graph_of is indexed by rank; each graph's ranks are one
-//   contiguous range).
-//   A workgroup works on one item at a time, so all LDS structures share one
-//   region R, kept zero between items: the workgroup bitmap R[0, bitmap_words)
-//   with its low array after it, the workgroup hash table R[0, hash_slots),
-//   the wave hash tables R[w kWaveCap, +kWaveCap) and the wave windows
-//   R[w (window_words + kWaveMaxDeg), ...) with their low arrays (wave-bitmap
-//   and wave-pull items).  Bitmaps clear what they set; tables and low arrays
-//   are cleared after use.  Pull streams one list at a time here: the packed
-//   stream's registers would halve this kernel's occupancy for every item.
+// Two phases per workgroup, each draining one queue:
+//   1. workgroup items, popped by the whole workgroup (heaviest kinds first):
+//      merge-lane chunks of kLaneItem edges (one per thread), hash vertices,
+//      bitmap vertices, workgroup-pull in-edge chunks;
+//   2. wave items, popped by each wave on its own (kWaveChunk per pop):
+//      merge-wave edges, wave-hash and wave-bitmap vertices, wave-pull
+//      in-edge chunks.  A wave never waits for the others, so short and long
+//      items mix without idling the workgroup.
+// Both queues keep contiguous per-XCD ranges: a batch graph's ranks are
+// contiguous, so each small graph stays in one XCD's L2 (dealing the wave
+// pops over the XCDs instead made bitmap counting of the 4,096-graph batch
+// 109 -> 168 ms).  The workgroup is capped at 64 VGPRs (8 waves per SIMD).
+// Every item's triangles are added to the counters of the graph owning its
+// vertex (the source, or for pull the target: the graphs are disjoint, so
+// its in-edges come from its own graph; graph_of is indexed by rank; each
+// graph's ranks are one contiguous range).
+//   All LDS structures share one region R, kept zero between items.  Phase 1
+// uses the workgroup bitmap R[0, bitmap_words) with its low array after it,
+// or the workgroup hash table R[0, hash_slots).  In phase 2 wave w owns the
+// slot R[w S, (w + 1) S), S = window_words + kWaveMaxDeg + kPackUnit / 4:
+// its hash table or window bitmap at the start, the window's low array after
+// the window, and its packed-streaming heads array at the end (the heads need
+// no clearing: phase 1, which needs zeros there, is over by then).  Bitmaps
+// clear what they set; tables and low arrays are cleared after use.
 constexpr uint32_t kBatchBlock = 512;
 constexpr uint32_t kLaneItem = kBatchBlock;
 constexpr uint32_t kWavesPerBatchBlock = kBatchBlock / 64;
@@ -1733,7 +1719,7 @@ __device__ __forceinline__ void lane_flush_graph(unsigned long long *per_graph, 
   }
 }
 
-__global__ __launch_bounds__(kBatchBlock) void k_batch(
+__global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
     DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vh,
     const uint32_t *__restrict__ vb, const uint32_t *__restrict__ vw,
@@ -1750,83 +1736,85 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(
   if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
-  uint32_t *const TW = R + wave * kWaveCap;                       // wave hash table
-  uint32_t *const BW = R + wave * (window_words + kWaveMaxDeg);   // wave window
-  uint32_t *const LW = BW + window_words;                         // its low array
-  const uint32_t window = window_words * 16u;
   const uint32_t nl = ctrl->len[0], nwv = ctrl->len[1], nh = ctrl->len[2], nb = ctrl->len[3],
                  nhw = ctrl->len[4], nbw = ctrl->len[5], np = ctrl->len[6], nwp = ctrl->len[7];
-  const uint32_t i_lane = (nl + kLaneItem - 1) / kLaneItem;
-  const uint32_t i_wave = (nwv + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
-  const uint32_t i_hw = (nhw + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
-  const uint32_t i_bw = (nbw + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
-  const uint32_t i_wp = (nwp + kWavesPerBatchBlock - 1) / kWavesPerBatchBlock;
-  const uint32_t b1 = i_lane, b2 = b1 + i_wave, b3 = b2 + i_hw, b3w = b3 + i_bw,
-                 b3p = b3w + i_wp, b4 = b3p + nh, b5 = b4 + nb, total = b5 + np;
-  QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<false> acc;
-  uint32_t it;
-  while (block_pop(ctrl->queue[3], total, cur, &s_first, &s_count, it)) {
-    if (it < b1) {  // merge-lane chunk
-      const uint32_t i = it * kLaneItem + tid;
-      const bool valid = i < nl;
-      uint32_t c[4] = {0, 0, 0, 0};
-      uint32_t gid = 0;
-      if (valid) {
-        const uint32_t e = lane_list[i];
-        gid = graph_of[g.src[e]];
-        merge_lane_edge<false>(g, e, c);
-      }
-      lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
-    } else if (it < b2) {  // merge-wave chunk
-      const uint32_t i = (it - b1) * kWavesPerBatchBlock + wave;
-      if (i < nwv) {
-        const uint32_t e = __builtin_amdgcn_readfirstlane(wave_list[i]);
-        merge_wave_edge<false>(g, e, acc);
-        acc.flush_graph(per_graph, graph_of[g.src[e]]);
-      }
-    } else if (it < b3) {  // wave-hash chunk
-      const uint32_t i = (it - b2) * kWavesPerBatchBlock + wave;
-      if (i < nhw) {
-        const uint32_t a = __builtin_amdgcn_readfirstlane(vw[i]);
-        hash_vertex_wave<false>(g, a, TW, table, tune, acc);
+  // phase 1: workgroup items [lane chunks | hash | bitmap | pull chunks]
+  {
+    const uint32_t b1 = (nl + kLaneItem - 1) / kLaneItem, b2 = b1 + nh, b3 = b2 + nb,
+                   total = b3 + np;
+    QueueCursor cur{blockIdx.x & 7u, 0u};
+    uint32_t it;
+    while (block_pop(ctrl->queue[3], total, cur, &s_first, &s_count, it)) {
+      if (it < b1) {  // merge-lane chunk
+        const uint32_t i = it * kLaneItem + tid;
+        const bool valid = i < nl;
+        uint32_t c[4] = {0, 0, 0, 0};
+        uint32_t gid = 0;
+        if (valid) {
+          const uint32_t e = lane_list[i];
+          gid = graph_of[g.src[e]];
+          merge_lane_edge<false>(g, e, c);
+        }
+        lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
+      } else if (it < b2) {  // hash vertex
+        const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b1]);
+        hash_vertex_block<false>(g, a, R, table, tune, acc);
         acc.flush_graph(per_graph, graph_of[a]);
-#pragma unroll
-        for (uint32_t k = lane; k < kWaveCap; k += 64) TW[k] = 0u;
-      }
-    } else if (it < b3w) {  // wave-bitmap chunk
-      const uint32_t i = (it - b3) * kWavesPerBatchBlock + wave;
-      if (i < nbw) {
-        const uint32_t a = __builtin_amdgcn_readfirstlane(vwb[i]);
-        bitmap_vertex_wave<false>(g, a, BW, LW, window, table, tune, acc);
+        for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
+      } else if (it < b3) {  // bitmap vertex
+        const uint32_t a = __builtin_amdgcn_readfirstlane(vb[it - b2]);
+        bitmap_vertex_block<false>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
         acc.flush_graph(per_graph, graph_of[a]);
-        LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
-      }
-    } else if (it < b3p) {  // wave-pull chunk
-      const uint32_t i = (it - b3w) * kWavesPerBatchBlock + wave;
-      if (i < nwp) {
-        const uint32_t j0 = __builtin_amdgcn_readfirstlane(vwp[i]);
-        const uint32_t v = pull_vertex_wave<false, false>(g, j0, BW, LW, window, nullptr, table,
-                                                          acc);
+        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
+      } else {  // workgroup-pull chunk
+        const uint32_t j0 = __builtin_amdgcn_readfirstlane(vp[it - b3]);
+        const uint32_t v =
+            pull_vertex_block<false>(g, j0, R, R + bitmap_words, &s_nlow, table, tune, acc);
         acc.flush_graph(per_graph, graph_of[v]);
-        LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
+        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
       }
-    } else if (it < b4) {  // hash vertex
-      const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b3p]);
-      hash_vertex_block<false>(g, a, R, table, tune, acc);
-      acc.flush_graph(per_graph, graph_of[a]);
-      for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
-    } else if (it < b5) {  // bitmap vertex
-      const uint32_t a = __builtin_amdgcn_readfirstlane(vb[it - b4]);
-      bitmap_vertex_block<false>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
-      acc.flush_graph(per_graph, graph_of[a]);
-      for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
-    } else {  // workgroup-pull chunk
-      const uint32_t j0 = __builtin_amdgcn_readfirstlane(vp[it - b5]);
-      const uint32_t v =
-          pull_vertex_block<false>(g, j0, R, R + bitmap_words, &s_nlow, table, tune, acc);
-      acc.flush_graph(per_graph, graph_of[v]);
-      for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
+    }
+  }
+  // phase 2: wave items [merge-wave | wave hash | wave bitmap | wave pull]
+  {
+    const uint32_t slot = window_words + kWaveMaxDeg + kPackUnit / 4u;
+    uint32_t *const W = R + wave * slot;
+    uint32_t *const LW = W + window_words;
+    uint64_t *const HW = (uint64_t *)(LW + kWaveMaxDeg);
+    const uint32_t window = window_words * 16u;
+    const uint32_t w1 = nwv, w2 = w1 + nhw, w3 = w2 + nbw, total = w3 + nwp;
+    QueueCursor cur{(blockIdx.x + wave) & 7u, 0u};
+    while (true) {
+      uint32_t first = 0, count = 0;
+      if (lane == 0 && !queue_pop(ctrl->queue[4], total, kWaveChunk, cur, first, count))
+        count = 0;
+      first = __shfl(first, 0, 64);
+      count = __shfl(count, 0, 64);
+      if (count == 0) break;
+      for (uint32_t it = first; it < first + count; it++) {
+        if (it < w1) {  // merge-wave edge
+          const uint32_t e = __builtin_amdgcn_readfirstlane(wave_list[it]);
+          merge_wave_edge<false>(g, e, acc);
+          acc.flush_graph(per_graph, graph_of[g.src[e]]);
+        } else if (it < w2) {  // wave-hash vertex
+          const uint32_t a = __builtin_amdgcn_readfirstlane(vw[it - w1]);
+          hash_vertex_wave<false>(g, a, W, table, tune, acc);
+          acc.flush_graph(per_graph, graph_of[a]);
+#pragma unroll
+          for (uint32_t k = lane; k < kWaveCap; k += 64) W[k] = 0u;
+        } else if (it < w3) {  // wave-bitmap vertex
+          const uint32_t a = __builtin_amdgcn_readfirstlane(vwb[it - w2]);
+          bitmap_vertex_wave<false>(g, a, W, LW, window, table, tune, acc);
+          acc.flush_graph(per_graph, graph_of[a]);
+          LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
+        } else {  // wave-pull chunk
+          const uint32_t j0 = __builtin_amdgcn_readfirstlane(vwp[it - w3]);
+          const uint32_t v = pull_vertex_wave<false>(g, j0, W, LW, window, HW, table, acc);
+          acc.flush_graph(per_graph, graph_of[v]);
+          LW[lane] = 0u;
+        }
+      }
     }
   }
 }
@@ -2058,8 +2046,10 @@ hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *
 static size_t batch_region_words(const TuneParams &t) {
   const size_t bitmap = (size_t)((1u << t.bitmap_span_log2) / 16u) + kLowCap;
   const size_t hash = (size_t)4u << t.hash_max_log2;
-  const size_t waves = (size_t)kWavesPerBatchBlock *
-                       std::max<size_t>(kWaveCap, (1u << t.wave_window_log2) / 16u + kWaveMaxDeg);
+  // phase 2: one slot per wave (hash table or window + low array, then heads)
+  const size_t wave_slot = std::max<size_t>(kWaveCap, (1u << t.wave_window_log2) / 16u) +
+                           kWaveMaxDeg + kPackUnit / 4u;
+  const size_t waves = (size_t)kWavesPerBatchBlock * wave_slot;
   return std::max(bitmap, std::max(hash, waves));
 }
 
@@ -2075,7 +2065,8 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
                      (uint32_t *)per_graph, 4ull * k, 0u);
   STC_TRY(hipGetLastError());
   const size_t lds = batch_lds_bytes(hp.tune);
-  const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (150 * 1024) / lds));
+  // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
+  const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
   hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
                      cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vb, L.vw,
                      L.vwb, L.vp, L.vwp, graph_of, per_graph,
