@@ -232,6 +232,31 @@ def test_batch_small_vs_oracle(ctx):
         assert got == want, mode
 
 
+@pytest.mark.parametrize("seed", [11, 12])
+def test_batch_mixed_strategy_tables(ctx, seed):
+    """The batched kernel with tables mixing merge, hash, bitmap and pull
+    bucket by bucket: every item kind of both phases (workgroup items, then
+    wave items) in one launch, sharing the LDS region, counts every triangle
+    once.  The batch holds small graphs, an RMAT graph with long lists and a
+    hub graph whose targets span many pull chunks."""
+    rng = np.random.default_rng(seed)
+    b = stc.gen_batch(24, n_hi=20000, d_hi=48.0, seed=seed)
+    graphs = [b.graph(i) for i in range(b.k)]
+    graphs.append(stc.gen_rmat(13, 16, 0.57, 0.19, 0.19, 0.3, seed))
+    graphs.append(stc.gen_hub(100000, 400000, 4, 1.2, 0.6, 1000, 0.3, seed))
+    want = [oracle_counts(g.n, g.u, g.v, g.sign) for g in graphs]
+    total = (sum(w[0] for w in want), sum(w[1] for w in want))
+    ctx.prepare_batch(graphs)
+    for _ in range(3):
+        t = rng.choice([stc.TABLE[s] for s in ("merge", "hash", "bitmap", "pull")],
+                       256).astype(np.uint8)
+        got = ctx.count_with_table(t)
+        say("batch mixed", seed, got)
+        assert got == total
+    say("batch mixed per graph")
+    assert ctx.count_batch(graphs, dispatch="pull") == want
+
+
 def test_batch_rejects_out_of_range_id(ctx):
     graphs = [stc.Graph(u=[0, 1], v=[1, 2], sign=[1, 1], n=3),
               stc.Graph(u=[0, 3], v=[1, 2], sign=[1, 1], n=3)]
