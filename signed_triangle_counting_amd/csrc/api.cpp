@@ -64,6 +64,17 @@ int hip_fail(stc_ctx *c, hipError_t e, const char *where) {
   return fail(c, STC_E_HIP, m);
 }
 
+// Pinned staging for an upload of `lines` host lines: true when the lines go
+// through the staging ring.  A host that refuses the pinned buffers (a locked-
+// memory limit, say) gets the direct pageable copies instead of an error.
+bool use_staging(stc_ctx *c, uint64_t lines) {
+  if (lines < kStageMinLines) return false;
+  if (c->ring.ensure() == hipSuccess) return true;
+  c->ring.release();
+  (void)hipGetLastError();
+  return false;
+}
+
 // Column words are (rank << 1) | neg with 0xFFFFFFFF as padding, and the
 // bitmap probe relies on 2^32 - 2 rank exceeding twice the largest rank window
 // (2^19): both hold for ranks below 2^31 - 2^20.
@@ -339,11 +350,10 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
         (e = c->bb.in_v.ensure(mm * 4)) != hipSuccess ||
         (e = c->bb.in_s.ensure(mm)) != hipSuccess)
       return hip_fail(c, e, "upload alloc");
-    if (g->m >= kStageMinLines && (e = c->ring.ensure()) != hipSuccess)
-      return hip_fail(c, e, "staging alloc");
+    const bool staged = use_staging(c, g->m);
     e = hipStreamSynchronize(c->st);
     const auto t0 = std::chrono::steady_clock::now();
-    if (g->m >= kStageMinLines) {
+    if (staged) {
       if (e == hipSuccess)
         e = c->ring.upload({EdgePiece{g->u, g->v, g->sign, g->m}}, c->bb.in_u.as<uint32_t>(),
                            c->bb.in_v.as<uint32_t>(), c->bb.in_s.as<int8_t>(), c->st);
@@ -564,14 +574,13 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
   uint32_t *d_vbase = (uint32_t *)(d_ebase + (k + 1));
   uint32_t *d_bad = d_vbase + (k + 1);
   float ms_up = 0.f;
-  if (m >= kStageMinLines && (e = c->ring.ensure()) != hipSuccess)
-    return hip_fail(c, e, "staging alloc");
+  const bool staged = use_staging(c, m);
   e = hipStreamSynchronize(c->st);
   const auto t0 = std::chrono::steady_clock::now();
   // find the first graph with lines for the contiguous fast path
   uint32_t first = 0;
   while (first < k && graphs[first].m == 0) first++;
-  if (e == hipSuccess && m >= kStageMinLines) {
+  if (e == hipSuccess && staged) {
     // the graphs back to back through the pinned staging buffers
     std::vector<EdgePiece> pieces;
     pieces.reserve(k);
