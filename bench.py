@@ -72,6 +72,18 @@ def traffic_for(kernel, workload):
         return None
 
 
+def cpu_model():
+    """The host CPU's model name, as lscpu prints it (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(g, budget_s):
     """The oracle's merge count (tc_merge_sign_enu.cpp:131-145 restated) over
     every stride-th source vertex; a first pass over 1/64 of the vertices
@@ -91,6 +103,8 @@ def cpu_baseline(g, budget_s):
         "value": s["edges"] / s["seconds"] if s["seconds"] > 0 else None,
         "unit": "oriented edges/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": "oracle merge count (OpenMP, %d threads) over source vertices v %% %d == %d: "
                   "%d of %d oriented edges in %.1f s; oracle CSR build (%.1f s) not included"
