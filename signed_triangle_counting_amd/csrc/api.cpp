@@ -389,6 +389,7 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   c->g.tin_off = c->bb.tin_off.as<uint32_t>();
   c->g.tin = c->bb.tin.as<InEdge>();
   c->g.dcls = c->bb.dcls.as<uint8_t>();
+  c->g.smask = c->bb.smask.as<uint32_t>();
   c->prepared = true;
   c->batch_ready = false;
   std::memset(&c->stats, 0, sizeof(c->stats));
@@ -631,6 +632,7 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
   c->g.tin_off = c->bb.tin_off.as<uint32_t>();
   c->g.tin = c->bb.tin.as<InEdge>();
   c->g.dcls = c->bb.dcls.as<uint8_t>();
+  c->g.smask = c->bb.smask.as<uint32_t>();
   c->batch_ready = true;
   c->batch_k = k;
   std::memset(&c->stats, 0, sizeof(c->stats));

@@ -362,11 +362,12 @@ __device__ __forceinline__ uint32_t lds_append(bool pred, uint32_t *s_len) {
 
 // Strategy of every edge from its stored bucket byte; merge edges go to the
 // workgroup's list segments, hash and bitmap edges flag their source vertex
-// (bit 0 hash, bit 1 bitmap), pull edges their target vertex (bit 2).  Edges
-// are sorted by source, so the lanes of one source form runs: each run's
-// first lane sets the run's flag bits with one atomic, only when they are not
-// set yet; a pull target's flag is set once, by the first edge that sees it
-// clear.
+// (bit 0 hash, bit 1 bitmap).  Edges are sorted by source, so the lanes of one
+// source form runs: each run's first lane sets the run's flag bits with one
+// atomic, only when they are not set yet.  Pull edges need nothing here: the
+// vertex-list pass finds their targets from each target's suffix-class mask
+// (a per-edge flag on the target cost one device-scope atomic per pull edge:
+// 23 ms of a 92 ms batch of 605 M edges).
 __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
                                                      const uint8_t *__restrict__ bk8,
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
     uint8_t s = S_SKIP;
-    uint32_t a = 0xFFFFFFFFu, v = 0;
+    uint32_t a = 0xFFFFFFFFu;
     if (e < g.m) {
       a = g.src[e];
       const uint32_t bk = bk8[e];
@@ -405,10 +406,8 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
               return span_of(g.col, g.off[b], g.off[b + 1]);
             },
             (int)bk);
-        if (s == S_PULL) v = g.col[e] >> 1;
       }
     }
-    if (s == S_PULL && !(vflag[v] & 4u)) atomicOr(&vflag[v], 4u);  // pull: flag the target
     uint32_t p = lds_append(s == S_MERGE_LANE, &s_len[0]);
     if (s == S_MERGE_LANE) lane_out[p] = (uint32_t)e;
     p = lds_append(s == S_MERGE_WAVE, &s_len[1]);
@@ -458,7 +457,7 @@ __global__ __launch_bounds__(256) void k_list_compact(const uint32_t *__restrict
 // Vertices with hash / bitmap / pull work go to six lists in ascending rank:
 // workgroup hash, workgroup bitmap, wave hash, wave bitmap, workgroup pull,
 // wave pull (vertices with at most kWaveMaxDeg out-edges use the wave
-// kernels).  For pull the flagged vertex is the target v whose list is
+// kernels).  For pull the listed vertex is the target v whose list is
 // staged, and its entries are chunks of its in-edge range [tin_off[v],
 // tin_off[v + 1]): one entry per kPullBlockChunk (workgroup) or
 // kPullWaveChunk (wave) in-edges, holding the chunk's first position, so a
@@ -472,24 +471,52 @@ constexpr int kVLists = 6;
 constexpr uint32_t kPullBlockChunk = 2048;
 constexpr uint32_t kPullWaveChunk = 256;
 
-// Entries of each list a vertex adds (0 or 1, pull lists one per chunk).
-__device__ __forceinline__ void vertex_lists_of(uint32_t v, uint32_t n, const uint32_t *vflag,
-                                                const uint32_t *off, const uint32_t *tin_off,
-                                                uint32_t c[kVLists], uint32_t &f) {
+// The strategy table by target class: pull_src[cb] has bit c set when bucket
+// (c, cb) is binned to pull (built in LDS by every workgroup of the pass;
+// any_pull_in reads it back after the barrier that follows).
+__device__ __forceinline__ void pull_sources_of(const Ctrl *ctrl, uint16_t *pull_src) {
+  const uint32_t cb = threadIdx.x;
+  if (cb < (uint32_t)kBucketSide) {
+    uint32_t msk = 0;
+    for (int c = 0; c < kBucketSide; c++)
+      if (ctrl->table[c * kBucketSide + cb] == S_PULL) msk |= 1u << c;
+    pull_src[cb] = (uint16_t)msk;
+  }
+}
+
+__device__ __forceinline__ bool any_pull_in(const uint16_t *pull_src) {
+  uint32_t any = 0;
+#pragma unroll
+  for (int c = 0; c < kBucketSide; c++) any |= pull_src[c];
+  return any != 0;
+}
+
+// Entries of each list a vertex adds (0 or 1, pull lists one per chunk).  A
+// target is a pull vertex when some in-edge lies in a bucket binned to pull
+// (its suffix-class mask meets pull_src of its own class) and its list fits
+// the pull kernels -- exactly when some in-edge's strategy is pull, since
+// the fit test reads the target alone.
+__device__ __forceinline__ void vertex_lists_of(uint32_t v, const DevCsr &g, const uint32_t *vflag,
+                                                const uint16_t *pull_src, bool any_pull,
+                                                const TuneParams &t, uint32_t c[kVLists],
+                                                uint32_t &f) {
   f = 0;
-  bool small = false;
+  bool small = false, pull = false;
   uint32_t indeg = 0;
-  if (v < n) {
+  if (v < g.n) {
     f = vflag[v];
-    if (f) small = off[v + 1] - off[v] <= kWaveMaxDeg;
-    if (f & 4u) indeg = tin_off[v + 1] - tin_off[v];
+    const uint32_t cb = any_pull ? (uint32_t)g.dcls[v] : 0xFFu;  // no pull bucket: no reads
+    if (cb != 0xFFu && (g.smask[v] & pull_src[cb]))
+      pull = pull_fits(t, (int)cb, [&] { return span_of(g.col, g.off[v], g.off[v + 1]); });
+    if (f || pull) small = g.off[v + 1] - g.off[v] <= kWaveMaxDeg;
+    if (pull) indeg = g.tin_off[v + 1] - g.tin_off[v];
   }
   c[0] = (f & 1u) && !small;
   c[1] = (f & 2u) && !small;
   c[2] = (f & 1u) && small;
   c[3] = (f & 2u) && small;
-  c[4] = (f & 4u) && !small ? (indeg + kPullBlockChunk - 1) / kPullBlockChunk : 0u;
-  c[5] = (f & 4u) && small ? (indeg + kPullWaveChunk - 1) / kPullWaveChunk : 0u;
+  c[4] = pull && !small ? (indeg + kPullBlockChunk - 1) / kPullBlockChunk : 0u;
+  c[5] = pull && small ? (indeg + kPullWaveChunk - 1) / kPullWaveChunk : 0u;
 }
 
 __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
@@ -508,20 +535,25 @@ __host__ __device__ inline uint64_t vlist_stride(uint32_t n, uint64_t m) {
   return (uint64_t)n + 1 + m / kPullWaveChunk + 1;
 }
 
-__global__ __launch_bounds__(256) void k_vcount(const uint32_t *__restrict__ vflag,
-                                                const uint32_t *__restrict__ off,
-                                                const uint32_t *__restrict__ tin_off, uint32_t n,
+__global__ __launch_bounds__(256) void k_vcount(DevCsr g, const Ctrl *ctrl,
+                                                const DispatchParams *P,
+                                                const uint32_t *__restrict__ vflag,
                                                 uint32_t *__restrict__ tile_cnt) {
   __shared__ uint32_t s_cnt[kVLists];
+  __shared__ uint16_t s_pull[kBucketSide];
+  __shared__ TuneParams tune;
   const uint32_t tid = threadIdx.x, ntiles = gridDim.x;
   if (tid < kVLists) s_cnt[tid] = 0;
+  if (tid == 0) tune = P->tune;
+  pull_sources_of(ctrl, s_pull);
   __syncthreads();
+  const bool any_pull = any_pull_in(s_pull);
   uint32_t cnt[kVLists] = {0, 0, 0, 0, 0, 0};
   const uint32_t base = blockIdx.x * kVTile;
   for (uint32_t r = 0; r < kVTile; r += 256) {
     uint32_t c[kVLists];
     uint32_t f;
-    vertex_lists_of(base + r + tid, n, vflag, off, tin_off, c, f);
+    vertex_lists_of(base + r + tid, g, vflag, s_pull, any_pull, tune, c, f);
 #pragma unroll
     for (int L = 0; L < kVLists; L++) cnt[L] += c[L];
   }
@@ -534,15 +566,16 @@ __global__ __launch_bounds__(256) void k_vcount(const uint32_t *__restrict__ vfl
 
 // tile_pos: exclusive scan of tile_cnt (list-major, kVLists * ntiles
 // entries); list L goes to vlist + L * stride.
-__global__ __launch_bounds__(256) void k_vscatter(uint32_t *__restrict__ vflag,
-                                                  const uint32_t *__restrict__ off,
-                                                  const uint32_t *__restrict__ tin_off, uint32_t n,
+__global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams *P,
+                                                  uint32_t *__restrict__ vflag,
                                                   const uint32_t *__restrict__ tile_cnt,
                                                   const uint32_t *__restrict__ tile_pos,
                                                   uint32_t *__restrict__ vlist, uint64_t stride,
                                                   Ctrl *ctrl) {
   __shared__ uint32_t s_wcnt[kVLists][4];  // [list][wave] entries in this round
   __shared__ uint32_t s_run[kVLists];      // entries of earlier rounds in this tile
+  __shared__ uint16_t s_pull[kBucketSide];
+  __shared__ TuneParams tune;
   const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   const uint32_t ntiles = gridDim.x, b = blockIdx.x;
   uint32_t tile_off[kVLists];
@@ -550,12 +583,16 @@ __global__ __launch_bounds__(256) void k_vscatter(uint32_t *__restrict__ vflag,
   for (int L = 0; L < kVLists; L++)
     tile_off[L] = tile_pos[L * ntiles + b] - tile_pos[L * ntiles];
   if (tid < kVLists) s_run[tid] = 0;
+  if (tid == 0) tune = P->tune;
+  pull_sources_of(ctrl, s_pull);
+  __syncthreads();
+  const bool any_pull = any_pull_in(s_pull);
   const uint32_t base = b * kVTile;
   for (uint32_t r = 0; r < kVTile; r += 256) {
     const uint32_t v = base + r + tid;
     uint32_t c[kVLists];
     uint32_t f;
-    vertex_lists_of(v, n, vflag, off, tin_off, c, f);
+    vertex_lists_of(v, g, vflag, s_pull, any_pull, tune, c, f);
     uint32_t incl[kVLists];
 #pragma unroll
     for (int L = 0; L < kVLists; L++) {
@@ -573,7 +610,7 @@ __global__ __launch_bounds__(256) void k_vscatter(uint32_t *__restrict__ vflag,
           dst[0] = v;
         } else {
           const uint32_t chunk = L == 4 ? kPullBlockChunk : kPullWaveChunk;
-          const uint32_t t0 = tin_off[v];
+          const uint32_t t0 = g.tin_off[v];
           for (uint32_t k = 0; k < c[L]; k++) dst[k] = t0 + k * chunk;
         }
       }
@@ -1943,12 +1980,11 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
                      ctrl);
   STC_TRY(hipGetLastError());
   if (n > 0) {
-    hipLaunchKernelGGL(k_vcount, dim3(ntiles), dim3(256), 0, st, vf, g.off, g.tin_off, n,
-                       tile_cnt);
+    hipLaunchKernelGGL(k_vcount, dim3(ntiles), dim3(256), 0, st, g, ctrl, dp, vf, tile_cnt);
     STC_TRY(hipGetLastError());
     STC_TRY(scan_exclusive_u32(tile_cnt, tile_pos, kVLists * (uint64_t)ntiles, cb.scan, st));
-    hipLaunchKernelGGL(k_vscatter, dim3(ntiles), dim3(256), 0, st, vf, g.off, g.tin_off, n,
-                       tile_cnt, tile_pos, out.vh, vstride, ctrl);
+    hipLaunchKernelGGL(k_vscatter, dim3(ntiles), dim3(256), 0, st, g, dp, vf, tile_cnt,
+                       tile_pos, out.vh, vstride, ctrl);
     STC_TRY(hipGetLastError());
   }
   if (marks) STC_TRY(hipEventRecord(marks[1], st));

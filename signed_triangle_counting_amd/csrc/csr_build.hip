@@ -347,33 +347,55 @@ __global__ void k_tkeys(const uint32_t *__restrict__ col, uint64_t m, uint64_t *
 }
 
 // In-edge records of the sorted keys (edge e = (u -> v): the part of N+(u)
-// after v and the edge's sign), and in-degrees (targets are sorted: one
-// atomic per run of equal targets in a wave).
+// after v and the edge's sign), in-degrees, and each target's suffix-class
+// mask: bit c of smask[v] is set when some in-edge's part after v is
+// non-empty and of class min(ceil_log2 len, 15) -- the source side of the
+// binning bucket, so a counting pass can tell from the target alone whether
+// any in-edge falls in a given bucket.  Targets are sorted, so a wave sees
+// runs of equal targets: a run with a different target on both sides inside
+// the wave holds all of that target's in-edges and is stored directly; a run
+// touching lane 0 or lane 63 may continue in a neighbouring wave and is
+// added (or or-ed) atomically into the zeroed arrays.
 __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m,
                          const uint32_t *__restrict__ col, const uint32_t *__restrict__ src,
                          const uint32_t *__restrict__ off, InEdge *__restrict__ tin,
-                         uint32_t *__restrict__ tdeg) {
+                         uint32_t *__restrict__ tdeg, uint32_t *__restrict__ smask) {
   const uint32_t lane = __lane_id();
   const uint64_t mround = (m + 63) & ~63ull;  // keep waves converged
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mround;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const bool in = i < m;
-    uint32_t v = 0xFFFFFFFFu;
+    uint32_t v = 0xFFFFFFFFu, cbit = 0;
     if (in) {
       const uint64_t k = key[i];
       v = (uint32_t)(k >> 32);
       const uint32_t e = (uint32_t)k;
-      const uint32_t end = off[src[e] + 1];
-      tin[i] = InEdge{e + 1u, ((end - e - 1u) << 1) | (col[e] & 1u)};
+      const uint32_t len = off[src[e] + 1] - e - 1u;
+      tin[i] = InEdge{e + 1u, (len << 1) | (col[e] & 1u)};
+      const uint32_t c = len <= 1 ? 0u : 32u - (uint32_t)__clz(len - 1u);
+      cbit = len ? 1u << min(c, 15u) : 0u;
     }
     const uint32_t prev = (uint32_t)__shfl_up((int)v, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || prev != v);
     const uint64_t valid = __ballot(in);
+    uint64_t has[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) has[c] = __ballot((cbit >> c) & 1u);
     if (in && ((heads >> lane) & 1u)) {
       const uint64_t above = heads & ~((2ull << lane) - 1ull);
       const uint64_t run = (above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull) &
                            ~((1ull << lane) - 1ull);
-      atomicAdd(&tdeg[v], (uint32_t)__popcll(valid & run));
+      const uint32_t cnt = (uint32_t)__popcll(valid & run);
+      uint32_t mask = 0;
+#pragma unroll
+      for (int c = 0; c < 16; c++) mask |= (has[c] & run) ? 1u << c : 0u;
+      if (lane > 0 && !(run >> 63)) {
+        tdeg[v] = cnt;
+        smask[v] = mask;
+      } else {
+        atomicAdd(&tdeg[v], cnt);
+        if (mask) atomicOr(&smask[v], mask);
+      }
     }
   }
 }
@@ -647,11 +669,14 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
   hipLaunchKernelGGL(k_tkeys, dim3(grid_for(m, T)), dim3(T), 0, st, bb.col.as<uint32_t>(), m, kA);
   STC_TRY(hipGetLastError());
   STC_TRY(radix_sort_u64(&kA, &kB, m, shifts_for(32, nb), bb.hist, bb.scr, st));
+  STC_TRY(bb.smask.ensure(((uint64_t)n + 1) * 4));
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1, T)), dim3(T), 0, st, deg, (uint64_t)n + 1,
                      0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1, T)), dim3(T), 0, st,
+                     bb.smask.as<uint32_t>(), (uint64_t)n + 1, 0u);
   hipLaunchKernelGGL(k_tsplit, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m,
                      bb.col.as<uint32_t>(), bb.src.as<uint32_t>(), bb.off.as<uint32_t>(),
-                     bb.tin.as<InEdge>(), deg);
+                     bb.tin.as<InEdge>(), deg, bb.smask.as<uint32_t>());
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(deg, bb.tin_off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
   // algorithmic bytes and max out-degree

@@ -98,6 +98,9 @@ struct DevCsr {
   const uint32_t *tin_off = nullptr;  // n + 1: in-edges of each target rank
   const InEdge *tin = nullptr;        // m: in-edges grouped by target, ascending e
   const uint8_t *dcls = nullptr;      // n: min(ceil_log2 d+(v), 15), 0xFF when d+(v) = 0
+  // n: bit c set when some in-edge (u -> v) has a non-empty part of N+(u)
+  // after v, of length len with min(ceil_log2 len, 15) = c
+  const uint32_t *smask = nullptr;
   // Per-vertex four-class counters by rank (4 n words: T0..T3), credited to
   // each triangle's lowest-ranked vertex by the four-class kernels; nullptr
   // when per-vertex counts are not requested.
@@ -177,7 +180,7 @@ struct BuildBuffers {
   DevBuf in_u, in_v, in_s;
   DevBuf keyA, keyB, flag, pos;
   DevBuf deg, vkeyA, vkeyB, rank, perm, graph_of;
-  DevBuf col, src, off, fdeg, tin, tin_off, dcls;
+  DevBuf col, src, off, fdeg, tin, tin_off, dcls, smask;
   DevBuf hist, misc;
   ScanScratch scr;
 };
