@@ -611,7 +611,11 @@ __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams
         } else {
           const uint32_t chunk = L == 4 ? kPullBlockChunk : kPullWaveChunk;
           const uint32_t t0 = g.tin_off[v];
-          for (uint32_t k = 0; k < c[L]; k++) dst[k] = t0 + k * chunk;
+          uint32_t *dst_t = dst + 2 * stride;  // the entries' targets
+          for (uint32_t k = 0; k < c[L]; k++) {
+            dst[k] = t0 + k * chunk;
+            dst_t[k] = v;
+          }
         }
       }
     }
@@ -729,6 +733,25 @@ struct Acc {
       wave_add(&per_graph[2 * gid + 1], c[1]);
     }
     c[0] = c[1] = c[2] = c[3] = 0;
+  }
+};
+
+// Batched counting: a wave's totals go to its graph's counters only when the
+// wave moves on to another graph (and once at the end), not after every item.
+// Consecutive items of a queue are neighbouring ranks, mostly of one graph,
+// and every wave of an XCD works near the same ranks, so a flush per item put
+// two device-scope atomics per item on the same two counters.  to(gid) is
+// called before an item of graph gid.
+struct GraphAcc {
+  Acc<false> acc;
+  uint32_t cur = 0xFFFFFFFFu;
+  __device__ __forceinline__ void to(unsigned long long *per_graph, uint32_t gid) {
+    if (gid == cur) return;
+    if (cur != 0xFFFFFFFFu) acc.flush_graph(per_graph, cur);
+    cur = gid;
+  }
+  __device__ __forceinline__ void finish(unsigned long long *per_graph) {
+    if (cur != 0xFFFFFFFFu) acc.flush_graph(per_graph, cur);
   }
 };
 
@@ -1495,19 +1518,20 @@ __device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, 
 // Pull: stage the target's list N+(v) and count the pull in-edges (u -> v)
 // of one chunk of v's in-edge range by streaming the part of N+(u) after v;
 // each edge's triangles belong to u.  The chunk starts at in-edge position
-// j0 and holds at most `chunk` in-edges; v is the target of its first edge.
+// j0 and holds at most `chunk` in-edges; the list entry carries v beside j0,
+// so staging need not wait for a load of the chunk's first edge.
 // An in-edge is pull iff its bucket (|N+(u) after v|, d+(v)) says pull (v was
 // listed only if its list fits).
 struct PullChunk {
   uint32_t v, t0, t1;
 };
 
-__device__ __forceinline__ PullChunk pull_chunk(const DevCsr &g, uint32_t j0, uint32_t chunk) {
+__device__ __forceinline__ PullChunk pull_chunk(const DevCsr &g, uint32_t j0, uint32_t v,
+                                                uint32_t chunk) {
   PullChunk c;
   c.t0 = j0;
-  const uint32_t e = __builtin_amdgcn_readfirstlane(g.tin[j0].pos) - 1u;
-  c.v = __builtin_amdgcn_readfirstlane(g.col[e] >> 1);
-  c.t1 = min(j0 + chunk, __builtin_amdgcn_readfirstlane(g.tin_off[c.v + 1]));
+  c.v = v;
+  c.t1 = min(j0 + chunk, __builtin_amdgcn_readfirstlane(g.tin_off[v + 1]));
   return c;
 }
 
@@ -1530,35 +1554,34 @@ __device__ __forceinline__ void pull_stream(const DevCsr &g, uint32_t first, uin
     stream_packed<CLS>(g, first, t1, step, PullMeta{g}, keep, probe, H, acc);
 }
 
-// Both return the chunk's target v (its graph owns the chunk's triangles).
+// The chunk of target v's in-edges starting at position j0.
 template <bool CLS>
-__device__ __forceinline__ uint32_t pull_vertex_block(const DevCsr &g, uint32_t j0, uint32_t *B,
-                                                      uint32_t *L, uint32_t *s_nlow,
-                                                      const uint8_t *table,
-                                                      const TuneParams &tune, Acc<CLS> &acc) {
+__device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, uint32_t v,
+                                                  uint32_t *B, uint32_t *L, uint32_t *s_nlow,
+                                                  const uint8_t *table, const TuneParams &tune,
+                                                  Acc<CLS> &acc) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nwaves = blockDim.x >> 6;
-  const PullChunk c = pull_chunk(g, j0, kPullBlockChunk);
+  const PullChunk c = pull_chunk(g, j0, v, kPullBlockChunk);
   const Staged st = stage_block(g, c.v, B, L, s_nlow, 1u << tune.bitmap_span_log2);
   st.with_probe(B, L, [&](const auto &probe) {
     pull_stream<CLS, false>(g, c.t0 + wave, c.t1, nwaves, st.d, table, probe, nullptr, acc);
   });
   unstage_block(g, st, B);
-  return c.v;
 }
 
 template <bool CLS>
-__device__ __forceinline__ uint32_t pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t *B,
-                                                     uint32_t *L, uint32_t window, uint64_t *H,
-                                                     const uint8_t *table, Acc<CLS> &acc) {
-  const PullChunk c = pull_chunk(g, j0, kPullWaveChunk);
+__device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t v,
+                                                 uint32_t *B, uint32_t *L, uint32_t window,
+                                                 uint64_t *H, const uint8_t *table,
+                                                 Acc<CLS> &acc) {
+  const PullChunk c = pull_chunk(g, j0, v, kPullWaveChunk);
   const StagedWave sw = stage_wave(g, c.v, B, L, window);
   const Staged &st = sw.st;
   st.with_probe(B, L, [&](const auto &probe) {
     pull_stream<CLS, true>(g, c.t0, c.t1, 1u, st.d, table, probe, H, acc);
   });
   unstage_wave(sw, B);
-  return c.v;
 }
 
 // Pop one item for the whole workgroup from an XCD queue.
@@ -1635,11 +1658,13 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
 // One workgroup per vertex: LDS bitmap (zeroed once per workgroup) followed
 // by the kLowCap-word low array.  Push (PULL = false) stages sources with
 // more than kWaveMaxDeg out-edges; a pull item is a chunk of the in-edges of
-// such a target (the list entry is the chunk's first in-edge position).
+// such a target (the list entry is the chunk's first in-edge position, vtarget
+// the matching entry's target; nullptr for push).
 template <bool CLS, bool PULL>
 __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                          const DispatchParams *P,
                                                          const uint32_t *__restrict__ vlist,
+                                                         const uint32_t *__restrict__ vtarget,
                                                          uint32_t bitmap_words) {
   extern __shared__ uint32_t B[];
   uint32_t *L = B + bitmap_words;
@@ -1657,7 +1682,8 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
   while (block_pop(ctrl->queue[PULL ? 5 : 1], len, cur, &s_first, &s_count, first)) {
     const uint32_t x = __builtin_amdgcn_readfirstlane(vlist[first]);
     if constexpr (PULL)
-      pull_vertex_block<CLS>(g, x, B, L, &s_nlow, table, tune, acc);
+      pull_vertex_block<CLS>(g, x, __builtin_amdgcn_readfirstlane(vtarget[first]), B, L, &s_nlow,
+                             table, tune, acc);
     else
       bitmap_vertex_block<CLS>(g, x, B, L, &s_nlow, table, tune, acc);
   }
@@ -1671,6 +1697,7 @@ template <bool CLS, bool PULL>
 __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
                                                      const uint32_t *__restrict__ vlist,
+                                                     const uint32_t *__restrict__ vtarget,
                                                      uint32_t window_words) {
   extern __shared__ uint32_t lds[];
   __shared__ uint8_t table[kBuckets];
@@ -1700,7 +1727,8 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
     for (uint32_t it = 0; it < count; it++) {
       const uint32_t x = __builtin_amdgcn_readfirstlane(vlist[first + it]);
       if constexpr (PULL)
-        pull_vertex_wave<CLS>(g, x, B, L, window, s_heads + wave * 64u, table, acc);
+        pull_vertex_wave<CLS>(g, x, __builtin_amdgcn_readfirstlane(vtarget[first + it]), B, L,
+                              window, s_heads + wave * 64u, table, acc);
       else
         bitmap_vertex_wave<CLS>(g, x, B, L, window, table, tune, acc);
     }
@@ -1758,10 +1786,8 @@ __device__ __forceinline__ void lane_flush_graph(unsigned long long *per_graph, 
 
 __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
     DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
-    const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vh,
-    const uint32_t *__restrict__ vb, const uint32_t *__restrict__ vw,
-    const uint32_t *__restrict__ vwb, const uint32_t *__restrict__ vp,
-    const uint32_t *__restrict__ vwp, const uint32_t *__restrict__ graph_of,
+    const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vlist, uint64_t vstride,
+    const uint32_t *__restrict__ graph_of,
     unsigned long long *__restrict__ per_graph, uint32_t bitmap_words, uint32_t hash_slots,
     uint32_t window_words, uint32_t region_words) {
   extern __shared__ uint32_t R[];
@@ -1775,7 +1801,8 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
   for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
   const uint32_t nl = ctrl->len[0], nwv = ctrl->len[1], nh = ctrl->len[2], nb = ctrl->len[3],
                  nhw = ctrl->len[4], nbw = ctrl->len[5], np = ctrl->len[6], nwp = ctrl->len[7];
-  Acc<false> acc;
+  GraphAcc ga;
+  Acc<false> &acc = ga.acc;
   // phase 1: workgroup items [lane chunks | hash | bitmap | pull chunks]
   {
     const uint32_t b1 = (nl + kLaneItem - 1) / kLaneItem, b2 = b1 + nh, b3 = b2 + nb,
@@ -1795,20 +1822,20 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
         }
         lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
       } else if (it < b2) {  // hash vertex
-        const uint32_t a = __builtin_amdgcn_readfirstlane(vh[it - b1]);
+        const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[it - b1]);
+        ga.to(per_graph, graph_of[a]);
         hash_vertex_block<false>(g, a, R, table, tune, acc);
-        acc.flush_graph(per_graph, graph_of[a]);
         for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
       } else if (it < b3) {  // bitmap vertex
-        const uint32_t a = __builtin_amdgcn_readfirstlane(vb[it - b2]);
+        const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[vstride + it - b2]);
+        ga.to(per_graph, graph_of[a]);
         bitmap_vertex_block<false>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
-        acc.flush_graph(per_graph, graph_of[a]);
         for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
       } else {  // workgroup-pull chunk
-        const uint32_t j0 = __builtin_amdgcn_readfirstlane(vp[it - b3]);
-        const uint32_t v =
-            pull_vertex_block<false>(g, j0, R, R + bitmap_words, &s_nlow, table, tune, acc);
-        acc.flush_graph(per_graph, graph_of[v]);
+        const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[4 * vstride + it - b3]);
+        const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[6 * vstride + it - b3]);
+        ga.to(per_graph, graph_of[v]);
+        pull_vertex_block<false>(g, j0, v, R, R + bitmap_words, &s_nlow, table, tune, acc);
         for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
       }
     }
@@ -1832,28 +1859,30 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
       for (uint32_t it = first; it < first + count; it++) {
         if (it < w1) {  // merge-wave edge
           const uint32_t e = __builtin_amdgcn_readfirstlane(wave_list[it]);
+          ga.to(per_graph, graph_of[g.src[e]]);
           merge_wave_edge<false>(g, e, acc);
-          acc.flush_graph(per_graph, graph_of[g.src[e]]);
         } else if (it < w2) {  // wave-hash vertex
-          const uint32_t a = __builtin_amdgcn_readfirstlane(vw[it - w1]);
+          const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[2 * vstride + it - w1]);
+          ga.to(per_graph, graph_of[a]);
           hash_vertex_wave<false>(g, a, W, table, tune, acc);
-          acc.flush_graph(per_graph, graph_of[a]);
 #pragma unroll
           for (uint32_t k = lane; k < kWaveCap; k += 64) W[k] = 0u;
         } else if (it < w3) {  // wave-bitmap vertex
-          const uint32_t a = __builtin_amdgcn_readfirstlane(vwb[it - w2]);
+          const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[3 * vstride + it - w2]);
+          ga.to(per_graph, graph_of[a]);
           bitmap_vertex_wave<false>(g, a, W, LW, window, table, tune, acc);
-          acc.flush_graph(per_graph, graph_of[a]);
           LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
         } else {  // wave-pull chunk
-          const uint32_t j0 = __builtin_amdgcn_readfirstlane(vwp[it - w3]);
-          const uint32_t v = pull_vertex_wave<false>(g, j0, W, LW, window, HW, table, acc);
-          acc.flush_graph(per_graph, graph_of[v]);
+          const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[5 * vstride + it - w3]);
+          const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[7 * vstride + it - w3]);
+          ga.to(per_graph, graph_of[v]);
+          pull_vertex_wave<false>(g, j0, v, W, LW, window, HW, table, acc);
           LW[lane] = 0u;
         }
       }
     }
   }
+  ga.finish(per_graph);
 }
 
 // Per-vertex counts from rank order to vertex-id order (perm: rank -> id).
@@ -1918,6 +1947,8 @@ namespace {
 
 struct BinOut {
   uint32_t *lane_list, *wave_list, *vh, *vb, *vw, *vwb, *vp, *vwp;
+  uint32_t *vpt, *vwpt;  // the target of each pull list entry
+  uint64_t vstride;      // list L (vh, vb, vw, vwb, vp, vwp, vpt, vwpt) at vh + L vstride
 };
 
 // Buffers: cb.gcount holds Ctrl; cb.lists the per-workgroup segments and the
@@ -1946,7 +1977,7 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   }
   const uint32_t ntiles = (uint32_t)(((uint64_t)n + kVTile - 1) / kVTile);
   const uint64_t vstride = vlist_stride(n, m);
-  STC_TRY(cb.vlist.ensure((kVLists * vstride + 2 * kVLists * (uint64_t)ntiles + 8) * 4));
+  STC_TRY(cb.vlist.ensure(((kVLists + 2) * vstride + 2 * kVLists * (uint64_t)ntiles + 8) * 4));
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
   uint32_t *lane_seg = cb.lists.as<uint32_t>();
   uint32_t *wave_seg = lane_seg + seg_words;
@@ -1956,12 +1987,15 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   uint32_t *blk_pos = blk_cnt + 2 * bin_grid;
   uint32_t *vf = cb.vflag.as<uint32_t>();
   out.vh = cb.vlist.as<uint32_t>();
+  out.vstride = vstride;
   out.vb = out.vh + vstride;
   out.vw = out.vb + vstride;
   out.vwb = out.vw + vstride;
   out.vp = out.vwb + vstride;
   out.vwp = out.vp + vstride;
-  uint32_t *tile_cnt = out.vwp + vstride;
+  out.vpt = out.vwp + vstride;  // list L's targets sit 2 strides after it
+  out.vwpt = out.vpt + vstride;
+  uint32_t *tile_cnt = out.vwpt + vstride;
   uint32_t *tile_pos = tile_cnt + kVLists * (uint64_t)ntiles;
 
   if (marks) STC_TRY(hipEventRecord(marks[0], st));
@@ -2027,16 +2061,16 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                        dp, L.vh);
     STC_TRY(mark(5));
     hipLaunchKernelGGL((k_bitmap_wave<true, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwb, bw_words);
+                       dp, L.vwb, nullptr, bw_words);
     STC_TRY(mark(6));
     hipLaunchKernelGGL((k_bitmap<true, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vb, bm_words);
+                       ctrl, dp, L.vb, nullptr, bm_words);
     STC_TRY(mark(7));
     hipLaunchKernelGGL((k_bitmap_wave<true, true>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwp, bw_words);
+                       dp, L.vwp, L.vwpt, bw_words);
     STC_TRY(mark(8));
     hipLaunchKernelGGL((k_bitmap<true, true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vp, bm_words);
+                       ctrl, dp, L.vp, L.vpt, bm_words);
     STC_TRY(mark(9));
   } else {
     hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, L.lane_list);
@@ -2049,16 +2083,16 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                        dp, L.vh);
     STC_TRY(mark(5));
     hipLaunchKernelGGL((k_bitmap_wave<false, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwb, bw_words);
+                       dp, L.vwb, nullptr, bw_words);
     STC_TRY(mark(6));
     hipLaunchKernelGGL((k_bitmap<false, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vb, bm_words);
+                       ctrl, dp, L.vb, nullptr, bm_words);
     STC_TRY(mark(7));
     hipLaunchKernelGGL((k_bitmap_wave<false, true>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwp, bw_words);
+                       dp, L.vwp, L.vwpt, bw_words);
     STC_TRY(mark(8));
     hipLaunchKernelGGL((k_bitmap<false, true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vp, bm_words);
+                       ctrl, dp, L.vp, L.vpt, bm_words);
     STC_TRY(mark(9));
   }
   return hipGetLastError();
@@ -2104,8 +2138,8 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
   // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
   const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
   hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
-                     cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vb, L.vw,
-                     L.vwb, L.vp, L.vwp, graph_of, per_graph,
+                     cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
+                     graph_of, per_graph,
                      (1u << hp.tune.bitmap_span_log2) / 16u,
                      4u << hp.tune.hash_max_log2, (1u << hp.tune.wave_window_log2) / 16u,
                      (uint32_t)batch_region_words(hp.tune));
