@@ -45,6 +45,8 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hub", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="count only the 4,096-graph batch (for profiling) and print its record")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--traffic-from", default=None,
                     help="write profiles/traffic.json from a rocprofv3 PMC directory and exit")
@@ -210,6 +212,10 @@ def main():
     import torch
     import signed_triangle_counting_amd as stc
 
+    if args.batch_only:
+        with stc.Counter(0) as ctx:
+            print(json.dumps({"batch": batch_workload(ctx, stc, args.steps)}), flush=True)
+        return
     wname = WORKLOADS[args.workload]
     g = stc.gen_workload(args.workload)
     ctx = stc.Counter(0)
