@@ -1435,14 +1435,15 @@ struct StagedWave {
   uint32_t o;
 };
 
-__device__ __forceinline__ StagedWave stage_wave(const DevCsr &g, uint32_t x, uint32_t *B,
-                                                 uint32_t *L, uint32_t window) {
+// The list col[la, ea) (offsets known to the caller).
+__device__ __forceinline__ StagedWave stage_wave_at(const DevCsr &g, uint32_t la, uint32_t ea,
+                                                    uint32_t *B, uint32_t *L, uint32_t window) {
   const uint32_t *__restrict__ col = g.col;
   const uint32_t lane = __lane_id();
   StagedWave sw;
   Staged &st = sw.st;
-  st.la = __builtin_amdgcn_readfirstlane(g.off[x]);
-  st.ea = __builtin_amdgcn_readfirstlane(g.off[x + 1]);
+  st.la = la;
+  st.ea = ea;
   st.d = st.ea - st.la;
   st.lo = __builtin_amdgcn_readfirstlane(col[st.la] >> 1);
   st.hi = __builtin_amdgcn_readfirstlane(col[st.ea - 1] >> 1);
@@ -1461,6 +1462,12 @@ __device__ __forceinline__ StagedWave stage_wave(const DevCsr &g, uint32_t x, ui
   st.nlow = (uint32_t)__popcll(__ballot(mine && !sw.in_w));
   wave_lds_sync();
   return sw;
+}
+
+__device__ __forceinline__ StagedWave stage_wave(const DevCsr &g, uint32_t x, uint32_t *B,
+                                                 uint32_t *L, uint32_t window) {
+  return stage_wave_at(g, __builtin_amdgcn_readfirstlane(g.off[x]),
+                       __builtin_amdgcn_readfirstlane(g.off[x + 1]), B, L, window);
 }
 
 __device__ __forceinline__ void unstage_wave(const StagedWave &sw, uint32_t *B) {
@@ -1570,16 +1577,16 @@ __device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, 
   unstage_block(g, st, B);
 }
 
+// One wave: the in-edges [j0, t1) of a target whose list is col[la, ea).
 template <bool CLS>
-__device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t v,
-                                                 uint32_t *B, uint32_t *L, uint32_t window,
-                                                 uint64_t *H, const uint8_t *table,
-                                                 Acc<CLS> &acc) {
-  const PullChunk c = pull_chunk(g, j0, v, kPullWaveChunk);
-  const StagedWave sw = stage_wave(g, c.v, B, L, window);
+__device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t t1,
+                                                 uint32_t la, uint32_t ea, uint32_t *B,
+                                                 uint32_t *L, uint32_t window, uint64_t *H,
+                                                 const uint8_t *table, Acc<CLS> &acc) {
+  const StagedWave sw = stage_wave_at(g, la, ea, B, L, window);
   const Staged &st = sw.st;
   st.with_probe(B, L, [&](const auto &probe) {
-    pull_stream<CLS, true>(g, c.t0, c.t1, 1u, st.d, table, probe, H, acc);
+    pull_stream<CLS, true>(g, j0, t1, 1u, st.d, table, probe, H, acc);
   });
   unstage_wave(sw, B);
 }
@@ -1623,6 +1630,42 @@ __global__ __launch_bounds__(kVertexBlock) void k_hash(DevCsr g, Ctrl *ctrl,
 
 // One wave per source vertex (source lists of at most kWaveMaxDeg entries).
 constexpr uint32_t kWaveChunk = 8;  // vertices per queue pop
+
+// Headers of the wave-pull chunks among the items of one queue pop, loaded
+// together so that an item does not start with a chain of dependent loads.
+// Items skip <= i < cnt are pull-list entries base + (i - skip).  Lane
+// kWaveChunk f + i of the result holds field f of item i: 0 the chunk's first
+// in-edge, 1 its target v, 2 off[v], 3 off[v + 1], 4 tin_off[v + 1], 5 v's
+// graph (0 without graph_of).  head_of reads one field back.
+constexpr uint32_t kHeadFields = 6;
+static_assert(kWaveChunk * kHeadFields <= 64, "a pop's headers fit one register");
+
+__device__ __forceinline__ uint32_t pull_heads(const DevCsr &g, const uint32_t *starts,
+                                               const uint32_t *targets,
+                                               const uint32_t *graph_of, uint32_t base,
+                                               uint32_t skip, uint32_t cnt) {
+  const uint32_t lane = __lane_id(), i = lane % kWaveChunk, f = lane / kWaveChunk;
+  uint32_t x = 0;
+  if (i >= skip && i < cnt && f < kHeadFields) {
+    const uint32_t k = base + (i - skip);
+    if (f == 0) {
+      x = starts[k];
+    } else {
+      const uint32_t v = targets[k];
+      const uint32_t *p = f == 2               ? g.off + v
+                          : f == 3             ? g.off + v + 1
+                          : f == 4             ? g.tin_off + v + 1
+                          : f == 5 && graph_of ? graph_of + v
+                                               : nullptr;
+      x = f == 1 ? v : p ? *p : 0u;
+    }
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t head_of(uint32_t heads, uint32_t field, uint32_t i) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)heads, (int)(field * kWaveChunk + i));
+}
 
 template <bool CLS>
 __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
@@ -1724,13 +1767,18 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
     first = __shfl(first, 0, 64);
     count = __shfl(count, 0, 64);
     if (count == 0) break;
+    const uint32_t heads =
+        PULL ? pull_heads(g, vlist, vtarget, nullptr, first, 0u, count) : 0u;
     for (uint32_t it = 0; it < count; it++) {
-      const uint32_t x = __builtin_amdgcn_readfirstlane(vlist[first + it]);
-      if constexpr (PULL)
-        pull_vertex_wave<CLS>(g, x, __builtin_amdgcn_readfirstlane(vtarget[first + it]), B, L,
-                              window, s_heads + wave * 64u, table, acc);
-      else
-        bitmap_vertex_wave<CLS>(g, x, B, L, window, table, tune, acc);
+      if constexpr (PULL) {
+        const uint32_t j0 = head_of(heads, 0, it);
+        pull_vertex_wave<CLS>(g, j0, min(j0 + kPullWaveChunk, head_of(heads, 4, it)),
+                              head_of(heads, 2, it), head_of(heads, 3, it), B, L, window,
+                              s_heads + wave * 64u, table, acc);
+      } else {
+        bitmap_vertex_wave<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first + it]), B, L,
+                                window, table, tune, acc);
+      }
     }
   }
   acc.flush(ctrl);
@@ -1856,6 +1904,12 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
       first = __shfl(first, 0, 64);
       count = __shfl(count, 0, 64);
       if (count == 0) break;
+      // headers of the pop's wave-pull items (items skip.. of the pop)
+      const uint32_t skip = first >= w3 ? 0u : min(w3 - first, count);
+      const uint32_t heads =
+          skip < count ? pull_heads(g, vlist + 5 * vstride, vlist + 7 * vstride, graph_of,
+                                    first + skip - w3, skip, count)
+                       : 0u;
       for (uint32_t it = first; it < first + count; it++) {
         if (it < w1) {  // merge-wave edge
           const uint32_t e = __builtin_amdgcn_readfirstlane(wave_list[it]);
@@ -1873,10 +1927,12 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
           bitmap_vertex_wave<false>(g, a, W, LW, window, table, tune, acc);
           LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
         } else {  // wave-pull chunk
-          const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[5 * vstride + it - w3]);
-          const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[7 * vstride + it - w3]);
-          ga.to(per_graph, graph_of[v]);
-          pull_vertex_wave<false>(g, j0, v, W, LW, window, HW, table, acc);
+          const uint32_t i = it - first;
+          const uint32_t j0 = head_of(heads, 0, i);
+          ga.to(per_graph, head_of(heads, 5, i));
+          pull_vertex_wave<false>(g, j0, min(j0 + kPullWaveChunk, head_of(heads, 4, i)),
+                                  head_of(heads, 2, i), head_of(heads, 3, i), W, LW, window,
+                                  HW, table, acc);
           LW[lane] = 0u;
         }
       }
