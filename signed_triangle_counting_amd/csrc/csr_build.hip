@@ -202,6 +202,8 @@ __global__ void k_dedupe_flag(const uint64_t *__restrict__ key, uint64_t m,
   }
 }
 
+constexpr int kHubRounds = 4;
+
 __global__ void k_compact_degree(const uint64_t *__restrict__ key,
                                  const uint32_t *__restrict__ flag,
                                  const uint32_t *__restrict__ pos, uint64_t m,
@@ -214,16 +216,30 @@ __global__ void k_compact_degree(const uint64_t *__restrict__ key,
        i += (uint64_t)gridDim.x * blockDim.x) {
     const bool in = i < m;
     bool keep = false;
-    uint32_t lo = 0xFFFFFFFFu;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
     if (in) {
       keep = flag[i] != 0;
       const uint64_t k = key[i];
       lo = (uint32_t)((k >> (nb + 1)) & mask);
       if (keep) {
         ukey[pos[i]] = k;
-        atomicAdd(&deg[(uint32_t)((k >> 1) & mask)], 1u);  // upper endpoints: scattered
+        hi = (uint32_t)((k >> 1) & mask);
       }
     }
+    // upper endpoints are scattered, but a hub is the upper endpoint of many
+    // lanes: a few leader rounds fold each of the commonest values of the wave
+    // into one atomic, and the lanes left add one each
+    uint64_t pending = __ballot(keep);
+#pragma unroll
+    for (int r = 0; r < kHubRounds; r++) {
+      if (!pending) break;
+      const uint32_t hv =
+          (uint32_t)__builtin_amdgcn_readlane((int)hi, (int)__builtin_ctzll(pending));
+      const uint64_t same = __ballot(hi == hv) & pending;
+      if (lane == (uint32_t)__builtin_ctzll(pending)) atomicAdd(&deg[hv], (uint32_t)__popcll(same));
+      pending &= ~same;
+    }
+    if ((pending >> lane) & 1u) atomicAdd(&deg[hi], 1u);
     // lower endpoints are sorted: one atomic per run of equal lo in the wave
     const uint32_t prev = (uint32_t)__shfl_up((int)lo, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || prev != lo);
@@ -340,10 +356,18 @@ __global__ void k_dcls(const uint32_t *__restrict__ off, uint32_t n, uint8_t *__
 
 // Keys of the transposed index: (target rank, edge id); sorted on the target
 // bits (stably, so edge ids stay ascending within a target).
-__global__ void k_tkeys(const uint32_t *__restrict__ col, uint64_t m, uint64_t *__restrict__ key) {
+// Each edge's in-edge record word, (length of N+(u) after v) << 1 | sign,
+// is written here in edge order (sequential reads), so the split after the
+// sort gathers one word per in-edge instead of src, then off, then col.
+__global__ void k_tkeys(const uint32_t *__restrict__ col, const uint32_t *__restrict__ src,
+                        const uint32_t *__restrict__ off, uint64_t m,
+                        uint64_t *__restrict__ key, uint32_t *__restrict__ rec) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    key[i] = ((uint64_t)(col[i] >> 1) << 32) | i;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t w = col[i];
+    key[i] = ((uint64_t)(w >> 1) << 32) | i;
+    rec[i] = ((off[src[i] + 1] - (uint32_t)i - 1u) << 1) | (w & 1u);
+  }
 }
 
 // In-edge records of the sorted keys (edge e = (u -> v): the part of N+(u)
@@ -357,8 +381,7 @@ __global__ void k_tkeys(const uint32_t *__restrict__ col, uint64_t m, uint64_t *
 // touching lane 0 or lane 63 may continue in a neighbouring wave and is
 // added (or or-ed) atomically into the zeroed arrays.
 __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m,
-                         const uint32_t *__restrict__ col, const uint32_t *__restrict__ src,
-                         const uint32_t *__restrict__ off, InEdge *__restrict__ tin,
+                         const uint32_t *__restrict__ rec, InEdge *__restrict__ tin,
                          uint32_t *__restrict__ tdeg, uint32_t *__restrict__ smask) {
   const uint32_t lane = __lane_id();
   const uint64_t mround = (m + 63) & ~63ull;  // keep waves converged
@@ -370,8 +393,9 @@ __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m,
       const uint64_t k = key[i];
       v = (uint32_t)(k >> 32);
       const uint32_t e = (uint32_t)k;
-      const uint32_t len = off[src[e] + 1] - e - 1u;
-      tin[i] = InEdge{e + 1u, (len << 1) | (col[e] & 1u)};
+      const uint32_t r = rec[e];
+      const uint32_t len = r >> 1;
+      tin[i] = InEdge{e + 1u, r};
       const uint32_t c = len <= 1 ? 0u : 32u - (uint32_t)__clz(len - 1u);
       cbit = len ? 1u << min(c, 15u) : 0u;
     }
@@ -666,7 +690,10 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
   // transposed index for the pull strategy: in-edge records grouped by target rank
   STC_TRY(bb.tin.ensure((m ? m : 1) * sizeof(InEdge)));
   STC_TRY(bb.tin_off.ensure(((uint64_t)n + 1) * 4));
-  hipLaunchKernelGGL(k_tkeys, dim3(grid_for(m, T)), dim3(T), 0, st, bb.col.as<uint32_t>(), m, kA);
+  // the dedupe flags are dead by now: their buffer holds the in-edge record words
+  uint32_t *rec = bb.flag.as<uint32_t>();
+  hipLaunchKernelGGL(k_tkeys, dim3(grid_for(m, T)), dim3(T), 0, st, bb.col.as<uint32_t>(),
+                     bb.src.as<uint32_t>(), bb.off.as<uint32_t>(), m, kA, rec);
   STC_TRY(hipGetLastError());
   STC_TRY(radix_sort_u64(&kA, &kB, m, shifts_for(32, nb), bb.hist, bb.scr, st));
   STC_TRY(bb.smask.ensure(((uint64_t)n + 1) * 4));
@@ -674,8 +701,7 @@ hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
                      0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1, T)), dim3(T), 0, st,
                      bb.smask.as<uint32_t>(), (uint64_t)n + 1, 0u);
-  hipLaunchKernelGGL(k_tsplit, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m,
-                     bb.col.as<uint32_t>(), bb.src.as<uint32_t>(), bb.off.as<uint32_t>(),
+  hipLaunchKernelGGL(k_tsplit, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m, rec,
                      bb.tin.as<InEdge>(), deg, bb.smask.as<uint32_t>());
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(deg, bb.tin_off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
