@@ -47,6 +47,8 @@ struct stc_ctx {
   hipEvent_t marks[kMarks] = {};
   DevBuf batch_counts, batch_vbase;
   UploadRing ring;  // pinned staging for host edge lists (allocated on first use)
+  bool fused_pull = false;  // last count was two-class: with workgroup-pull edges, one
+                            // launch counted both pull slots (its time in the last)
 };
 
 namespace {
@@ -448,6 +450,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
     out->balanced = res.t[0];
     out->unbalanced = res.t[1];
   }
+  c->fused_pull = !classes;
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
   c->stats.ms_bin = bins.back();
@@ -510,6 +513,10 @@ int stc_get_stats(stc_ctx *c, stc_stats *out) {
     c->stats.edges_bitmap = res.kern_edges[4] + res.kern_edges[5];
     c->stats.edges_pull = res.kern_edges[6] + res.kern_edges[7];
     for (int k = 0; k < kKernels; k++) c->stats.bytes_kernel[k] = 4.0 * (double)res.kern_words[k];
+    if (c->fused_pull && res.kern_edges[7] > 0) {  // one launch counted both pull slots
+      c->stats.bytes_kernel[7] += c->stats.bytes_kernel[6];
+      c->stats.bytes_kernel[6] = 0.0;
+    }
     c->kstats_pending = false;
   }
   *out = c->stats;
@@ -683,6 +690,7 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
   }
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->fused_pull = false;
   c->kstats_pending = true;
   if (res_out) *res_out = res;
   return STC_OK;
@@ -786,6 +794,7 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
   }
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->fused_pull = false;
   c->kstats_pending = true;
   c->stats.dispatch_used = (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
   c->have_classes = false;

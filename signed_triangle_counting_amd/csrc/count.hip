@@ -52,10 +52,12 @@ struct Ctrl {
   uint32_t work[8];  // dynamic work counters
   uint32_t pad[1];
   uint32_t queue[7][8][32];          // XCD work queues (workgroup hash,
-                                     // workgroup bitmap, wave hash, batch,
-                                     // wave bitmap, workgroup pull, wave
-                                     // pull): one counter per XCD, 128 bytes
-                                     // apart
+                                     // workgroup bitmap, wave hash, batch
+                                     // phase 1, wave bitmap or batch phase
+                                     // 2, workgroup pull, wave pull; a single
+                                     // graph's two-phase pull launch uses the
+                                     // last two): one counter per XCD, 128
+                                     // bytes apart
   unsigned long long t[4];           // class counters
   unsigned long long kern_edges[10]; // oriented edges counted per kernel
                                      // (launch order); slot 8: left out by
@@ -1741,13 +1743,16 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
                                                      const uint32_t *__restrict__ vlist,
                                                      const uint32_t *__restrict__ vtarget,
-                                                     uint32_t window_words) {
+                                                     uint32_t window_words, uint32_t defer) {
   extern __shared__ uint32_t lds[];
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
   __shared__ uint64_t s_heads[PULL ? 256 : 1];  // packed streaming, 64 per wave
   const uint32_t len = ctrl->len[PULL ? 7 : 5];
   if (len == 0) return;  // the whole grid: no vertex has this kernel's edges
+  // defer (pull): the two-phase launch that follows counts this list with the
+  // workgroup chunks whenever there are any
+  if (PULL && defer && ctrl->len[6] != 0) return;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = __lane_id();
   table[threadIdx.x] = ctrl->table[threadIdx.x];
@@ -1785,6 +1790,11 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 }
 
 // ---- batched counting: one persistent kernel over the union of k graphs ----
+// (A single graph's two-class count runs its pull items through the same
+// kernel as a batch of one: graph_of null, every total to per_graph[0..1],
+// pull_only set and queues q1, q2 that no other kernel of that count uses.
+// The two pull phases then share one launch, with no boundary and tail
+// between the workgroup and the wave chunks.)
 // Two phases per workgroup, each draining one queue:
 //   1. workgroup items, popped by the whole workgroup (heaviest kinds first):
 //      merge-lane chunks of kLaneItem edges (one per thread), hash vertices,
@@ -1837,18 +1847,25 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vlist, uint64_t vstride,
     const uint32_t *__restrict__ graph_of,
     unsigned long long *__restrict__ per_graph, uint32_t bitmap_words, uint32_t hash_slots,
-    uint32_t window_words, uint32_t region_words) {
+    uint32_t window_words, uint32_t region_words, uint32_t q1, uint32_t q2,
+    uint32_t pull_only) {
   extern __shared__ uint32_t R[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
   const uint32_t tid = threadIdx.x, lane = __lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t keep = pull_only ? 0u : 0xFFFFFFFFu;
+  const uint32_t nl = ctrl->len[0] & keep, nwv = ctrl->len[1] & keep, nh = ctrl->len[2] & keep,
+                 nb = ctrl->len[3] & keep, nhw = ctrl->len[4] & keep, nbw = ctrl->len[5] & keep,
+                 np = ctrl->len[6], nwp = ctrl->len[7];
+  if ((nl | nwv | nh | nb | nhw | nbw | np | nwp) == 0u) return;  // the whole grid: no items
+  // one graph's pull items: without workgroup chunks the wave-pull kernel,
+  // launched before, has counted the wave chunks
+  if (pull_only && np == 0u) return;
   if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
-  const uint32_t nl = ctrl->len[0], nwv = ctrl->len[1], nh = ctrl->len[2], nb = ctrl->len[3],
-                 nhw = ctrl->len[4], nbw = ctrl->len[5], np = ctrl->len[6], nwp = ctrl->len[7];
   GraphAcc ga;
   Acc<false> &acc = ga.acc;
   // phase 1: workgroup items [lane chunks | hash | bitmap | pull chunks]
@@ -1857,7 +1874,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
                    total = b3 + np;
     QueueCursor cur{blockIdx.x & 7u, 0u};
     uint32_t it;
-    while (block_pop(ctrl->queue[3], total, cur, &s_first, &s_count, it)) {
+    while (block_pop(ctrl->queue[q1], total, cur, &s_first, &s_count, it)) {
       if (it < b1) {  // merge-lane chunk
         const uint32_t i = it * kLaneItem + tid;
         const bool valid = i < nl;
@@ -1882,7 +1899,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
       } else {  // workgroup-pull chunk
         const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[4 * vstride + it - b3]);
         const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[6 * vstride + it - b3]);
-        ga.to(per_graph, graph_of[v]);
+        ga.to(per_graph, graph_of ? graph_of[v] : 0u);
         pull_vertex_block<false>(g, j0, v, R, R + bitmap_words, &s_nlow, table, tune, acc);
         for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
       }
@@ -1899,7 +1916,13 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
     QueueCursor cur{(blockIdx.x + wave) & 7u, 0u};
     while (true) {
       uint32_t first = 0, count = 0;
-      if (lane == 0 && !queue_pop(ctrl->queue[4], total, kWaveChunk, cur, first, count))
+      // a batch keeps each XCD on a contiguous range (its graphs stay in one
+      // L2); one graph's pops are dealt round-robin over the XCDs, which
+      // spreads the heavy top of the list (as in the single-graph wave kernels)
+      if (lane == 0 && !(pull_only ? queue_pop<true>(ctrl->queue[q2], total, kWaveChunk, cur,
+                                                     first, count)
+                                   : queue_pop<false>(ctrl->queue[q2], total, kWaveChunk, cur,
+                                                      first, count)))
         count = 0;
       first = __shfl(first, 0, 64);
       count = __shfl(count, 0, 64);
@@ -1988,7 +2011,7 @@ hipError_t init_kernel_attributes() {
   static bool done = false;
   if (done) return hipSuccess;
   const void *fns[] = {(const void *)k_bitmap<true, false>,      (const void *)k_bitmap<false, false>,
-                       (const void *)k_bitmap<true, true>,       (const void *)k_bitmap<false, true>,
+                       (const void *)k_bitmap<true, true>,
                        (const void *)k_bitmap_wave<true, false>, (const void *)k_bitmap_wave<false, false>,
                        (const void *)k_bitmap_wave<true, true>,  (const void *)k_bitmap_wave<false, true>,
                        (const void *)k_hash<true>,               (const void *)k_hash<false>,
@@ -2083,6 +2106,11 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
 
 }  // namespace
 
+hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
+                            CountBuffers &cb, const BinOut &L, const uint32_t *graph_of,
+                            unsigned long long *per_graph, int num_cus, hipStream_t st,
+                            uint32_t q1, uint32_t q2, bool pull_only);
+
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
                          int num_cus, hipStream_t st, const hipEvent_t *marks) {
@@ -2117,13 +2145,13 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                        dp, L.vh);
     STC_TRY(mark(5));
     hipLaunchKernelGGL((k_bitmap_wave<true, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwb, nullptr, bw_words);
+                       dp, L.vwb, nullptr, bw_words, 0u);
     STC_TRY(mark(6));
     hipLaunchKernelGGL((k_bitmap<true, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
                        ctrl, dp, L.vb, nullptr, bm_words);
     STC_TRY(mark(7));
     hipLaunchKernelGGL((k_bitmap_wave<true, true>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwp, L.vwpt, bw_words);
+                       dp, L.vwp, L.vwpt, bw_words, 0u);
     STC_TRY(mark(8));
     hipLaunchKernelGGL((k_bitmap<true, true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
                        ctrl, dp, L.vp, L.vpt, bm_words);
@@ -2139,16 +2167,19 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                        dp, L.vh);
     STC_TRY(mark(5));
     hipLaunchKernelGGL((k_bitmap_wave<false, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwb, nullptr, bw_words);
+                       dp, L.vwb, nullptr, bw_words, 0u);
     STC_TRY(mark(6));
     hipLaunchKernelGGL((k_bitmap<false, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
                        ctrl, dp, L.vb, nullptr, bm_words);
     STC_TRY(mark(7));
+    // pull: with workgroup chunks, both lists in one two-phase launch (a batch
+    // of one graph; its time is the workgroup slot's, the wave slot's about
+    // 0); without, the wave kernel alone (the two-phase launch exits at once)
     hipLaunchKernelGGL((k_bitmap_wave<false, true>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwp, L.vwpt, bw_words);
+                       dp, L.vwp, L.vwpt, bw_words, 1u);
     STC_TRY(mark(8));
-    hipLaunchKernelGGL((k_bitmap<false, true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vp, L.vpt, bm_words);
+    STC_TRY(launch_two_phase(g, hp, dp, cb, L, nullptr, (unsigned long long *)ctrl->t, num_cus,
+                             st, 5u, 6u, true));
     STC_TRY(mark(9));
   }
   return hipGetLastError();
@@ -2181,6 +2212,22 @@ static size_t batch_region_words(const TuneParams &t) {
 
 size_t batch_lds_bytes(const TuneParams &t) { return 4u * batch_region_words(t); }
 
+hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
+                            CountBuffers &cb, const BinOut &L, const uint32_t *graph_of,
+                            unsigned long long *per_graph, int num_cus, hipStream_t st,
+                            uint32_t q1, uint32_t q2, bool pull_only) {
+  const size_t lds = batch_lds_bytes(hp.tune);
+  // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
+  const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
+  hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
+                     cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
+                     graph_of, per_graph,
+                     (1u << hp.tune.bitmap_span_log2) / 16u,
+                     4u << hp.tune.hash_max_log2, (1u << hp.tune.wave_window_log2) / 16u,
+                     (uint32_t)batch_region_words(hp.tune), q1, q2, pull_only ? 1u : 0u);
+  return hipGetLastError();
+}
+
 hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                          CountBuffers &cb, const uint32_t *graph_of,
                          unsigned long long *per_graph, uint32_t k, int num_cus,
@@ -2190,16 +2237,7 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
   hipLaunchKernelGGL(k_fill_words, dim3(grid_for(4ull * k, 256, 4096)), dim3(256), 0, st,
                      (uint32_t *)per_graph, 4ull * k, 0u);
   STC_TRY(hipGetLastError());
-  const size_t lds = batch_lds_bytes(hp.tune);
-  // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
-  const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
-  hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
-                     cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
-                     graph_of, per_graph,
-                     (1u << hp.tune.bitmap_span_log2) / 16u,
-                     4u << hp.tune.hash_max_log2, (1u << hp.tune.wave_window_log2) / 16u,
-                     (uint32_t)batch_region_words(hp.tune));
-  return hipGetLastError();
+  return launch_two_phase(g, hp, dp, cb, L, graph_of, per_graph, num_cus, st, 3u, 4u, false);
 }
 
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_t st) {
