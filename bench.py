@@ -150,7 +150,7 @@ def traffic_from(pmc_dir, workload, out_path):
              "k_bitmap_wave<false, true>": "pull_wave",
              # a two-class count's workgroup and wave pull chunks share one
              # two-phase launch (the batch kernel, as a batch of one graph)
-             "k_batch": "pull"}
+             "k_two_phase": "pull"}
     out = {}
     try:
         with open(out_path) as f:

@@ -1842,7 +1842,7 @@ __device__ __forceinline__ void lane_flush_graph(unsigned long long *per_graph, 
   }
 }
 
-__global__ __launch_bounds__(kBatchBlock, 8) void k_batch(
+__global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
     DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vlist, uint64_t vstride,
     const uint32_t *__restrict__ graph_of,
@@ -2015,7 +2015,7 @@ hipError_t init_kernel_attributes() {
                        (const void *)k_bitmap_wave<true, false>, (const void *)k_bitmap_wave<false, false>,
                        (const void *)k_bitmap_wave<true, true>,  (const void *)k_bitmap_wave<false, true>,
                        (const void *)k_hash<true>,               (const void *)k_hash<false>,
-                       (const void *)k_batch};
+                       (const void *)k_two_phase};
   for (const void *f : fns)
     STC_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   done = true;
@@ -2219,7 +2219,7 @@ hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const Dis
   const size_t lds = batch_lds_bytes(hp.tune);
   // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
   const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
-  hipLaunchKernelGGL(k_batch, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
+  hipLaunchKernelGGL(k_two_phase, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
                      cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
                      graph_of, per_graph,
                      (1u << hp.tune.bitmap_span_log2) / 16u,
