@@ -567,7 +567,9 @@ __global__ __launch_bounds__(256) void k_vcount(DevCsr g, const Ctrl *ctrl,
 }
 
 // tile_pos: exclusive scan of tile_cnt (list-major, kVLists * ntiles
-// entries); list L goes to vlist + L * stride.
+// entries); list L goes to vlist + L * stride.  A tile that adds no entry to
+// any list has no flagged vertex either (every flag adds an entry), so its
+// workgroup returns at once: on a graph counted by merge alone every tile does.
 __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams *P,
                                                   uint32_t *__restrict__ vflag,
                                                   const uint32_t *__restrict__ tile_cnt,
@@ -581,9 +583,19 @@ __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams
   const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   const uint32_t ntiles = gridDim.x, b = blockIdx.x;
   uint32_t tile_off[kVLists];
+  uint32_t any = 0;
 #pragma unroll
-  for (int L = 0; L < kVLists; L++)
+  for (int L = 0; L < kVLists; L++) {
     tile_off[L] = tile_pos[L * ntiles + b] - tile_pos[L * ntiles];
+    any |= tile_cnt[L * ntiles + b];
+  }
+  if (any == 0u) {  // workgroup-uniform: no vertex of the tile is flagged or listed
+    if (b == ntiles - 1 && tid < kVLists) {
+      const uint32_t last = tid * ntiles + ntiles - 1;
+      ctrl->len[2 + tid] = tile_pos[last] + tile_cnt[last] - tile_pos[tid * ntiles];
+    }
+    return;
+  }
   if (tid < kVLists) s_run[tid] = 0;
   if (tid == 0) tune = P->tune;
   pull_sources_of(ctrl, s_pull);
