@@ -1,6 +1,7 @@
 """Train the learned dispatcher from timings the counter generates itself.
 
     python -m signed_triangle_counting_amd.train_dispatcher --out DIR [--tune-dir DIR]
+        [--samples DIR/dispatch_samples.npz] [--seed S] [--eval-model DIR]
 
 For each graph of a synthetic training set (seeds distinct from the benchmark
 workloads), and for training batches of small graphs counted together by the
@@ -155,37 +156,50 @@ def main():
     ap.add_argument("--lr", type=float, default=0.01)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tune-dir", default=None, help="tune.txt to apply first")
+    ap.add_argument("--samples", default=None,
+                    help="train from a saved dispatch_samples.npz instead of timing again")
+    ap.add_argument("--eval-model", default=None,
+                    help="only evaluate the dispatch_mlp.txt in this directory")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     ctx = stc.Counter(0, model_dir=args.tune_dir)
     t0 = time.time()
-    feats, costs, meta = [], [], []
-    print("collecting per-bucket strategy timings", flush=True)
     graphs = training_graphs(args.quick)
     batches = training_batches(args.quick)
     tuned = {}
-    for (name, thunk), is_batch in [(x, False) for x in graphs] + [(x, True) for x in batches]:
-        f, c, b, final = collect(ctx, name, thunk(), log=lambda s: print(s, flush=True),
-                                 batch=is_batch)
-        feats.append(f)
-        costs.append(c)
-        meta += [(name, x) for x in b]
-        tuned[name] = round(final, 4)
-    F = np.concatenate(feats)
-    Cst = np.concatenate(costs)
-    np.savez(os.path.join(args.out, "dispatch_samples.npz"), feat=F, cost=Cst)
-    print("samples %d collected in %.1fs" % (len(F), time.time() - t0), flush=True)
-    path = os.path.join(args.out, "dispatch_mlp.txt")
-    ctx.train_dispatcher(F, Cst, path, iters=args.iters, lr=args.lr, seed=args.seed)
-    print("trained ->", path, flush=True)
-    # agreement with the per-sample best strategy (training set)
-    ctx.load_models(args.out)
+    F = None
+    if args.eval_model:
+        ctx.load_models(args.eval_model)
+    else:
+        if args.samples:
+            with np.load(args.samples) as z:
+                F, Cst = z["feat"], z["cost"]
+            print("samples %d loaded from %s" % (len(F), args.samples), flush=True)
+        else:
+            feats, costs = [], []
+            print("collecting per-bucket strategy timings", flush=True)
+            for (name, thunk), is_batch in [(x, False) for x in graphs] + [(x, True) for x in batches]:
+                f, c, _, final = collect(ctx, name, thunk(), log=lambda s: print(s, flush=True),
+                                         batch=is_batch)
+                feats.append(f)
+                costs.append(c)
+                tuned[name] = round(final, 4)
+            F = np.concatenate(feats)
+            Cst = np.concatenate(costs)
+            np.savez(os.path.join(args.out, "dispatch_samples.npz"), feat=F, cost=Cst)
+            print("samples %d collected in %.1fs" % (len(F), time.time() - t0), flush=True)
+        path = os.path.join(args.out, "dispatch_mlp.txt")
+        ctx.train_dispatcher(F, Cst, path, iters=args.iters, lr=args.lr, seed=args.seed)
+        print("trained ->", path, flush=True)
+        ctx.load_models(args.out)
+    # every dispatch mode on every training graph and batch (and equal counts)
     rows = evaluate(ctx, graphs, log=lambda s: print(s, flush=True))
     rows += evaluate(ctx, batches, log=lambda s: print(s, flush=True), batch=True)
     for r in rows:
         r["coordinate_descent_table"] = tuned.get(r["graph"])
     with open(os.path.join(args.out, "dispatch_eval.json"), "w") as f:
-        json.dump({"samples": int(len(F)), "graphs": rows}, f, indent=1)
+        json.dump({"samples": int(len(F)) if F is not None else None, "graphs": rows}, f,
+                  indent=1)
     print("done in %.1fs" % (time.time() - t0), flush=True)
 
 
