@@ -880,7 +880,10 @@ extern "C" int stc_train_dispatcher(stc_ctx *c, const float *feat, const float *
     }
     for (int s = 0; s < kMlpOut; s++)
       r[(size_t)i * kMlpOut + s] = (cost[i * kMlpOut + s] - lo) / (hi - lo + 1e-9f);
-    w[i] = hi - lo;
+    // the relative time at stake: each cost is a whole count of the sample's
+    // graph, so lo is about that graph's best time; clamping at 1 (a 2x
+    // slowdown) keeps a few extreme buckets from drowning the small graphs
+    w[i] = std::min(1.f, (hi - lo) / (lo > 0.f ? lo : 1.f));
     wsum += w[i];
   }
   for (int i = 0; i < nsamp; i++) w[i] = (float)(w[i] / (wsum > 0 ? wsum : 1.0));
