@@ -110,9 +110,9 @@ typedef struct stc_stats {
                               wave-bitmap, workgroup-bitmap, wave-pull and
                               workgroup-pull kernel time (hipEvent on the
                               library stream), median over repeats; a
-                              two-class count with workgroup-pull edges runs
-                              both pull slots in one launch, reported in the
-                              last slot (wave-pull then reads about 0) */
+                              count with workgroup-pull edges runs both pull
+                              slots in one launch, reported in the last slot
+                              (wave-pull then reads about 0) */
   double bytes_kernel[8];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
                               oriented edges each of those kernels counted
                               (the same pull folding) */

@@ -47,7 +47,7 @@ struct stc_ctx {
   hipEvent_t marks[kMarks] = {};
   DevBuf batch_counts, batch_vbase;
   UploadRing ring;  // pinned staging for host edge lists (allocated on first use)
-  bool fused_pull = false;  // last count was two-class: with workgroup-pull edges, one
+  bool fused_pull = false;  // last count was one graph's: with workgroup-pull edges, one
                             // launch counted both pull slots (its time in the last)
 };
 
@@ -450,7 +450,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
     out->balanced = res.t[0];
     out->unbalanced = res.t[1];
   }
-  c->fused_pull = !classes;
+  c->fused_pull = true;
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
   c->stats.ms_bin = bins.back();

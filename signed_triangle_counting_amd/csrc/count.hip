@@ -755,17 +755,26 @@ struct Acc {
 // Consecutive items of a queue are neighbouring ranks, mostly of one graph,
 // and every wave of an XCD works near the same ranks, so a flush per item put
 // two device-scope atomics per item on the same two counters.  to(gid) is
-// called before an item of graph gid.
+// called before an item of graph gid.  The four-class variant serves one
+// graph only and adds its totals to the count's counters at the end.
+template <bool CLS>
 struct GraphAcc {
-  Acc<false> acc;
+  Acc<CLS> acc;
   uint32_t cur = 0xFFFFFFFFu;
   __device__ __forceinline__ void to(unsigned long long *per_graph, uint32_t gid) {
-    if (gid == cur) return;
-    if (cur != 0xFFFFFFFFu) acc.flush_graph(per_graph, cur);
-    cur = gid;
+    if constexpr (!CLS) {
+      if (gid == cur) return;
+      if (cur != 0xFFFFFFFFu) acc.flush_graph(per_graph, cur);
+      cur = gid;
+    }
   }
-  __device__ __forceinline__ void finish(unsigned long long *per_graph) {
-    if (cur != 0xFFFFFFFFu) acc.flush_graph(per_graph, cur);
+  // four-class counting is one graph's: its four counters sit in ctrl->t
+  __device__ __forceinline__ void finish(unsigned long long *per_graph, Ctrl *ctrl) {
+    if constexpr (CLS) {
+      acc.flush(ctrl);
+    } else if (cur != 0xFFFFFFFFu) {
+      acc.flush_graph(per_graph, cur);
+    }
   }
 };
 
@@ -1712,23 +1721,20 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   acc.flush(ctrl);
 }
 
-// One workgroup per vertex: LDS bitmap (zeroed once per workgroup) followed
-// by the kLowCap-word low array.  Push (PULL = false) stages sources with
-// more than kWaveMaxDeg out-edges; a pull item is a chunk of the in-edges of
-// such a target (the list entry is the chunk's first in-edge position, vtarget
-// the matching entry's target; nullptr for push).
-template <bool CLS, bool PULL>
+// One workgroup per source vertex with more than kWaveMaxDeg out-edges (push):
+// LDS bitmap (zeroed once per workgroup) followed by the kLowCap-word low
+// array.  (Workgroup pull chunks are counted by the two-phase kernel.)
+template <bool CLS>
 __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                          const DispatchParams *P,
                                                          const uint32_t *__restrict__ vlist,
-                                                         const uint32_t *__restrict__ vtarget,
                                                          uint32_t bitmap_words) {
   extern __shared__ uint32_t B[];
   uint32_t *L = B + bitmap_words;
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
-  const uint32_t len = ctrl->len[PULL ? 6 : 3];
+  const uint32_t len = ctrl->len[3];
   if (len == 0) return;  // the whole grid: no vertex has this kernel's edges
   if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
@@ -1736,14 +1742,9 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   uint32_t first;
-  while (block_pop(ctrl->queue[PULL ? 5 : 1], len, cur, &s_first, &s_count, first)) {
-    const uint32_t x = __builtin_amdgcn_readfirstlane(vlist[first]);
-    if constexpr (PULL)
-      pull_vertex_block<CLS>(g, x, __builtin_amdgcn_readfirstlane(vtarget[first]), B, L, &s_nlow,
+  while (block_pop(ctrl->queue[1], len, cur, &s_first, &s_count, first))
+    bitmap_vertex_block<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first]), B, L, &s_nlow,
                              table, tune, acc);
-    else
-      bitmap_vertex_block<CLS>(g, x, B, L, &s_nlow, table, tune, acc);
-  }
   acc.flush(ctrl);
 }
 
@@ -1802,11 +1803,12 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 }
 
 // ---- batched counting: one persistent kernel over the union of k graphs ----
-// (A single graph's two-class count runs its pull items through the same
-// kernel as a batch of one: graph_of null, every total to per_graph[0..1],
-// pull_only set and queues q1, q2 that no other kernel of that count uses.
-// The two pull phases then share one launch, with no boundary and tail
-// between the workgroup and the wave chunks.)
+// (A single graph's count runs its pull items through the same kernel as a
+// batch of one: graph_of null, pull_only set, queues q1, q2 that no other
+// kernel of that count uses; two-class totals go to per_graph[0..1], the
+// four-class variant (CLS) adds its classes to ctrl->t.  The two pull phases
+// then share one launch, with no boundary and tail between the workgroup and
+// the wave chunks.  Batches are two-class.)
 // Two phases per workgroup, each draining one queue:
 //   1. workgroup items, popped by the whole workgroup (heaviest kinds first):
 //      merge-lane chunks of kLaneItem edges (one per thread), hash vertices,
@@ -1854,6 +1856,7 @@ __device__ __forceinline__ void lane_flush_graph(unsigned long long *per_graph, 
   }
 }
 
+template <bool CLS>
 __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
     DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vlist, uint64_t vstride,
@@ -1878,8 +1881,8 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
   if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
-  GraphAcc ga;
-  Acc<false> &acc = ga.acc;
+  GraphAcc<CLS> ga;
+  Acc<CLS> &acc = ga.acc;
   // phase 1: workgroup items [lane chunks | hash | bitmap | pull chunks]
   {
     const uint32_t b1 = (nl + kLaneItem - 1) / kLaneItem, b2 = b1 + nh, b3 = b2 + nb,
@@ -1895,24 +1898,24 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
         if (valid) {
           const uint32_t e = lane_list[i];
           gid = graph_of[g.src[e]];
-          merge_lane_edge<false>(g, e, c);
+          if constexpr (!CLS) merge_lane_edge<false>(g, e, c);  // batches: two-class
         }
         lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
       } else if (it < b2) {  // hash vertex
         const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[it - b1]);
         ga.to(per_graph, graph_of[a]);
-        hash_vertex_block<false>(g, a, R, table, tune, acc);
+        hash_vertex_block<CLS>(g, a, R, table, tune, acc);
         for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
       } else if (it < b3) {  // bitmap vertex
         const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[vstride + it - b2]);
         ga.to(per_graph, graph_of[a]);
-        bitmap_vertex_block<false>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
+        bitmap_vertex_block<CLS>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
         for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
       } else {  // workgroup-pull chunk
         const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[4 * vstride + it - b3]);
         const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[6 * vstride + it - b3]);
         ga.to(per_graph, graph_of ? graph_of[v] : 0u);
-        pull_vertex_block<false>(g, j0, v, R, R + bitmap_words, &s_nlow, table, tune, acc);
+        pull_vertex_block<CLS>(g, j0, v, R, R + bitmap_words, &s_nlow, table, tune, acc);
         for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
       }
     }
@@ -1949,23 +1952,23 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
         if (it < w1) {  // merge-wave edge
           const uint32_t e = __builtin_amdgcn_readfirstlane(wave_list[it]);
           ga.to(per_graph, graph_of[g.src[e]]);
-          merge_wave_edge<false>(g, e, acc);
+          merge_wave_edge<CLS>(g, e, acc);
         } else if (it < w2) {  // wave-hash vertex
           const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[2 * vstride + it - w1]);
           ga.to(per_graph, graph_of[a]);
-          hash_vertex_wave<false>(g, a, W, table, tune, acc);
+          hash_vertex_wave<CLS>(g, a, W, table, tune, acc);
 #pragma unroll
           for (uint32_t k = lane; k < kWaveCap; k += 64) W[k] = 0u;
         } else if (it < w3) {  // wave-bitmap vertex
           const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[3 * vstride + it - w2]);
           ga.to(per_graph, graph_of[a]);
-          bitmap_vertex_wave<false>(g, a, W, LW, window, table, tune, acc);
+          bitmap_vertex_wave<CLS>(g, a, W, LW, window, table, tune, acc);
           LW[lane] = 0u;  // kWaveMaxDeg == 64 low words
         } else {  // wave-pull chunk
           const uint32_t i = it - first;
           const uint32_t j0 = head_of(heads, 0, i);
           ga.to(per_graph, head_of(heads, 5, i));
-          pull_vertex_wave<false>(g, j0, min(j0 + kPullWaveChunk, head_of(heads, 4, i)),
+          pull_vertex_wave<CLS>(g, j0, min(j0 + kPullWaveChunk, head_of(heads, 4, i)),
                                   head_of(heads, 2, i), head_of(heads, 3, i), W, LW, window,
                                   HW, table, acc);
           LW[lane] = 0u;
@@ -1973,7 +1976,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
       }
     }
   }
-  ga.finish(per_graph);
+  ga.finish(per_graph, ctrl);
 }
 
 // Per-vertex counts from rank order to vertex-id order (perm: rank -> id).
@@ -2022,12 +2025,11 @@ size_t wave_bitmap_lds_bytes(const TuneParams &t) {
 hipError_t init_kernel_attributes() {
   static bool done = false;
   if (done) return hipSuccess;
-  const void *fns[] = {(const void *)k_bitmap<true, false>,      (const void *)k_bitmap<false, false>,
-                       (const void *)k_bitmap<true, true>,
+  const void *fns[] = {(const void *)k_bitmap<true>,             (const void *)k_bitmap<false>,
                        (const void *)k_bitmap_wave<true, false>, (const void *)k_bitmap_wave<false, false>,
                        (const void *)k_bitmap_wave<true, true>,  (const void *)k_bitmap_wave<false, true>,
                        (const void *)k_hash<true>,               (const void *)k_hash<false>,
-                       (const void *)k_two_phase};
+                       (const void *)k_two_phase<true>,          (const void *)k_two_phase<false>};
   for (const void *f : fns)
     STC_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   done = true;
@@ -2121,7 +2123,7 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
 hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                             CountBuffers &cb, const BinOut &L, const uint32_t *graph_of,
                             unsigned long long *per_graph, int num_cus, hipStream_t st,
-                            uint32_t q1, uint32_t q2, bool pull_only);
+                            uint32_t q1, uint32_t q2, bool pull_only, bool classes);
 
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
@@ -2159,14 +2161,14 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL((k_bitmap_wave<true, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
                        dp, L.vwb, nullptr, bw_words, 0u);
     STC_TRY(mark(6));
-    hipLaunchKernelGGL((k_bitmap<true, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vb, nullptr, bm_words);
+    hipLaunchKernelGGL((k_bitmap<true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
+                       dp, L.vb, bm_words);
     STC_TRY(mark(7));
     hipLaunchKernelGGL((k_bitmap_wave<true, true>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
-                       dp, L.vwp, L.vwpt, bw_words, 0u);
+                       dp, L.vwp, L.vwpt, bw_words, 1u);
     STC_TRY(mark(8));
-    hipLaunchKernelGGL((k_bitmap<true, true>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vp, L.vpt, bm_words);
+    STC_TRY(launch_two_phase(g, hp, dp, cb, L, nullptr, (unsigned long long *)ctrl->t, num_cus,
+                             st, 5u, 6u, true, true));
     STC_TRY(mark(9));
   } else {
     hipLaunchKernelGGL(k_merge_lane<false>, dim3(grid), dim3(256), 0, st, g, ctrl, L.lane_list);
@@ -2181,8 +2183,8 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
     hipLaunchKernelGGL((k_bitmap_wave<false, false>), dim3(bw_grid), dim3(256), bw_lds, st, g, ctrl,
                        dp, L.vwb, nullptr, bw_words, 0u);
     STC_TRY(mark(6));
-    hipLaunchKernelGGL((k_bitmap<false, false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g,
-                       ctrl, dp, L.vb, nullptr, bm_words);
+    hipLaunchKernelGGL((k_bitmap<false>), dim3(bm_grid), dim3(kBitmapBlock), bm_lds, st, g, ctrl,
+                       dp, L.vb, bm_words);
     STC_TRY(mark(7));
     // pull: with workgroup chunks, both lists in one two-phase launch (a batch
     // of one graph; its time is the workgroup slot's, the wave slot's about
@@ -2191,7 +2193,7 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                        dp, L.vwp, L.vwpt, bw_words, 1u);
     STC_TRY(mark(8));
     STC_TRY(launch_two_phase(g, hp, dp, cb, L, nullptr, (unsigned long long *)ctrl->t, num_cus,
-                             st, 5u, 6u, true));
+                             st, 5u, 6u, true, false));
     STC_TRY(mark(9));
   }
   return hipGetLastError();
@@ -2227,11 +2229,13 @@ size_t batch_lds_bytes(const TuneParams &t) { return 4u * batch_region_words(t);
 hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                             CountBuffers &cb, const BinOut &L, const uint32_t *graph_of,
                             unsigned long long *per_graph, int num_cus, hipStream_t st,
-                            uint32_t q1, uint32_t q2, bool pull_only) {
+                            uint32_t q1, uint32_t q2, bool pull_only, bool classes) {
   const size_t lds = batch_lds_bytes(hp.tune);
   // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
   const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
-  hipLaunchKernelGGL(k_two_phase, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
+  // the four-class variant counts one graph's pull items only
+  auto kern = classes ? k_two_phase<true> : k_two_phase<false>;
+  hipLaunchKernelGGL(kern, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
                      cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
                      graph_of, per_graph,
                      (1u << hp.tune.bitmap_span_log2) / 16u,
@@ -2249,7 +2253,8 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
   hipLaunchKernelGGL(k_fill_words, dim3(grid_for(4ull * k, 256, 4096)), dim3(256), 0, st,
                      (uint32_t *)per_graph, 4ull * k, 0u);
   STC_TRY(hipGetLastError());
-  return launch_two_phase(g, hp, dp, cb, L, graph_of, per_graph, num_cus, st, 3u, 4u, false);
+  return launch_two_phase(g, hp, dp, cb, L, graph_of, per_graph, num_cus, st, 3u, 4u, false,
+                          false);
 }
 
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_t st) {
