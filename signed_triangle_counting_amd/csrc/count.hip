@@ -311,7 +311,10 @@ __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uin
 // ---- binning ----------------------------------------------------------------
 // Bucket histogram; every edge's bucket byte goes to bk8 (kNoBucket when
 // the edge cannot close a triangle) so the scatter pass need not recompute it.
-__global__ __launch_bounds__(256) void k_bin_hist(DevCsr g, Ctrl *ctrl,
+// (8 waves per SIMD, the launch's occupancy: the last workgroup's inference
+// would otherwise lift the register count past 64 and hold one workgroup per
+// CU back until another finished.)
+__global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl,
                                                   const DispatchParams *P,
                                                   uint8_t *__restrict__ bk8) {
   __shared__ uint32_t h[kBuckets];
