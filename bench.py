@@ -45,6 +45,8 @@ def parse_args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hub", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="measure the HBM read rate for sequential and random-segment reads")
     ap.add_argument("--batch-only", action="store_true",
                     help="count only the 4,096-graph batch (for profiling) and print its record")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -219,6 +221,10 @@ def main():
     import torch
     import signed_triangle_counting_amd as stc
 
+    if args.hbm_probe:
+        print(json.dumps({"hbm_read_tbps": stc.probe_read_bandwidth(0),
+                          "peak_tbps": HBM_PEAK_GBPS / 1000.0}), flush=True)
+        return
     if args.batch_only:
         with stc.Counter(0) as ctx:
             print(json.dumps({"batch": batch_workload(ctx, stc, args.steps)}), flush=True)

@@ -51,10 +51,11 @@ int stc_count_with_table(stc_ctx *ctx, const uint8_t table[STC_BUCKETS], int rep
 void stc_bucket_features(int bucket, uint32_t count, uint32_t n, uint64_t m, uint32_t k,
                          float out[STC_MLP_FEATURES]);
 
-/* Train the dispatcher MLP (6 -> 16 ReLU -> 3) on the device by full-batch
- * Adam over nsamp samples: feat (nsamp x 6) and cost (nsamp x 3, time of the
- * merge / hash / bitmap strategy for that sample, any unit).  The loss is the
- * softmax-expected normalised cost weighted by each sample's time at stake.
+/* Train the dispatcher MLP (7 -> 16 ReLU -> 4) on the device by full-batch
+ * Adam over nsamp samples: feat (nsamp x 7) and cost (nsamp x 4, time of the
+ * whole count with the sample's bucket under the merge / hash / bitmap / pull
+ * strategy).  The loss is the softmax-expected normalised cost, each sample
+ * weighted by its relative time at stake (max - min) / min, clamped at 1.
  * Deterministic for a given seed.  Writes the model file (text, checksummed)
  * to out_path and loads it into the context. */
 int stc_train_dispatcher(stc_ctx *ctx, const float *feat, const float *cost, int nsamp,
@@ -65,6 +66,15 @@ int stc_load_models(stc_ctx *ctx, const char *dir);
 
 /* Apply tune parameters given in the tune.txt text format. */
 int stc_set_tune(stc_ctx *ctx, const char *text);
+
+/* HBM read bandwidth of device `device` for the counting kernels' access
+ * patterns, in TB/s: out_tbps[0] a sequential read of a 4 GiB buffer;
+ * out_tbps[1..4] reads of random 128-byte-aligned segments of 128, 256, 512
+ * and 1024 bytes, 8 segments in flight per wave.  Best of 5 launches of about
+ * 4 GB each.  No counterpart in the reference: a measurement hook that gives
+ * the pull kernels' scattered suffix reads a ceiling for their own pattern.
+ * Allocates and frees 4 GiB of device memory. */
+int stc_probe_read_bandwidth(int device, double out_tbps[5]);
 
 #ifdef __cplusplus
 }

@@ -46,7 +46,7 @@ EXPORTED = [
     "stc_gen_er", "stc_gen_grid", "stc_gen_rmat", "stc_gen_hub", "stc_gen_batch",
     "stc_batch_free", "stc_gen_workload", "stc_gen_workload_batch",
     "stc_bucket_histogram", "stc_count_with_table", "stc_bucket_features",
-    "stc_train_dispatcher", "stc_load_models", "stc_set_tune",
+    "stc_train_dispatcher", "stc_load_models", "stc_set_tune", "stc_probe_read_bandwidth",
 ]
 
 TABLE = {"skip": 0, "merge": 1, "hash": 2, "bitmap": 3, "pull": 4}
@@ -167,6 +167,7 @@ def lib():
                                        C.c_float, C.c_uint64, C.c_char_p]
     L.stc_load_models.argtypes = [C.c_void_p, C.c_char_p]
     L.stc_set_tune.argtypes = [C.c_void_p, C.c_char_p]
+    L.stc_probe_read_bandwidth.argtypes = [C.c_int, P(C.c_double)]
     _lib_handle = L
     return L
 
@@ -304,6 +305,16 @@ def gen_batch(k, n_lo=64, n_hi=65536, d_lo=4.0, d_hi=64.0, p_lo=0.05, p_hi=0.5,
     _check(lib().stc_gen_batch(k, n_lo, n_hi, d_lo, d_hi, p_lo, p_hi, seed, threads,
                                C.byref(p)))
     return Batch(p)
+
+
+def probe_read_bandwidth(device=0):
+    """HBM read bandwidth (TB/s) of the device for a sequential read and for
+    random 128-byte-aligned segments of 128, 256, 512 and 1024 bytes
+    (stc_probe_read_bandwidth)."""
+    out = (C.c_double * 5)()
+    _check(lib().stc_probe_read_bandwidth(device, out))
+    keys = ("sequential", "segment_128B", "segment_256B", "segment_512B", "segment_1024B")
+    return {k: round(float(v), 3) for k, v in zip(keys, out)}
 
 
 def gen_workload_batch(k=4096, threads=0):

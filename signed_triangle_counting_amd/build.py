@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_obj")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["csr_build.hip", "count.hip", "mlp.hip"]
+HIP_SOURCES = ["csr_build.hip", "count.hip", "mlp.hip", "probe.hip"]
 HIP_HOST_SOURCES = ["api.cpp", "upload.cpp"]  # host code that calls the HIP runtime
 HOST_SOURCES = ["gen.cpp", "graph_io.cpp"]
 
