@@ -142,13 +142,19 @@ def test_edge_cases(ctx):
     assert ctx.count(stc.Graph(u=[0, 1, 0], v=[1, 2, 2], sign=[1, 1, 1], n=3)) == (1, 0)
 
 
-def test_classes_vs_oracle(ctx):
-    rng = np.random.default_rng(9)
-    u, v, s = random_graph(rng, 2000, 60000, dup=0.1)
-    want = oracle.Csr(2000, u, v, s).classes()
-    g = stc.Graph(u=u, v=v, sign=s, n=2000)
+@pytest.mark.parametrize("case", ["random", "rmat15"])
+def test_classes_vs_oracle(ctx, case):
+    """Four-class totals in every mode; the RMAT graph has targets of more than
+    64 out-edges, so pull runs workgroup and wave chunks in one launch."""
+    if case == "random":
+        rng = np.random.default_rng(9)
+        u, v, s = random_graph(rng, 2000, 60000, dup=0.1)
+        g = stc.Graph(u=u, v=v, sign=s, n=2000)
+    else:
+        g = stc.gen_rmat(15, 16, 0.57, 0.19, 0.19, 0.3, 77)
+    want = oracle.Csr(g.n, g.u, g.v, g.sign).classes()
     for mode in MODES:
-        say("classes", mode)
+        say("classes", case, mode)
         b, un = ctx.count(g, dispatch=mode, classes=True)
         assert ctx.classes() == want
         assert (b, un) == (want["t3"] + want["t1"], want["t2"] + want["t0"])
