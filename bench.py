@@ -148,7 +148,7 @@ def traffic_from(pmc_dir, workload, out_path):
     fetch, write = read("fetch"), read("write")
     names = {"k_merge_lane": "merge_lane", "k_merge_wave": "merge_wave",
              "k_hash_wave": "hash_wave", "k_hash<": "hash",
-             "k_bitmap_wave<false, false>": "bitmap_wave", "k_bitmap<false, false>": "bitmap",
+             "k_bitmap_wave<false, false>": "bitmap_wave", "k_bitmap<false>": "bitmap",
              "k_bitmap_wave<false, true>": "pull_wave",
              # a two-class count's workgroup and wave pull chunks share one
              # two-phase launch (the batch kernel, as a batch of one graph)
