@@ -7,8 +7,9 @@
 // read and the rest of the line ignored; u == v lines are dropped; n = max id
 // + 1 over kept lines.  The file is split into per-thread chunks at newline
 // boundaries as graph_io.h:117-133 does, but each chunk is parsed into its own
-// buffer and the buffers are joined in file order, so a dropped self-loop
-// leaves no hole (graph_io.h:147,159-161 reserve one and never fill it).
+// slice of the output and the slices are closed up in file order, so a
+// dropped self-loop leaves no hole (graph_io.h:147,159-161 reserve one and
+// never fill it).
 //
 // A buffer that starts with the "%%MatrixMarket" banner is read as a
 // MatrixMarket coordinate file instead, the input of the reference's GPU
@@ -40,14 +41,13 @@
 
 namespace {
 
-struct Chunk {
-  std::vector<uint32_t> u, v;
-  std::vector<int8_t> s;
-  uint32_t max_id = 0;
-  bool any = false;
-};
-
-void parse_range(const char *p, const char *end, Chunk &out) {
+// One chunk's entries, written straight into its slice of the output arrays
+// (at most one entry per line, so the slice holds the chunk's newline count
+// plus one).  Returns the entries kept; max_id gets the largest kept id.
+uint64_t parse_range(const char *p, const char *end, uint32_t *ou, uint32_t *ov, int8_t *os,
+                     uint32_t &max_id) {
+  uint64_t k = 0;
+  uint32_t mx = 0;
   while (p < end) {
     if (*p == '\n' || *p == '\r') {
       p++;
@@ -72,13 +72,26 @@ void parse_range(const char *p, const char *end, Chunk &out) {
     while (p < end && *p >= '0' && *p <= '9') p++;
     while (p < end && *p != '\n') p++;
     if (a == b) continue;
-    out.u.push_back(a);
-    out.v.push_back(b);
-    out.s.push_back(s);
-    uint32_t hi = a > b ? a : b;
-    if (hi > out.max_id) out.max_id = hi;
-    out.any = true;
+    ou[k] = a;
+    ov[k] = b;
+    os[k] = s;
+    k++;
+    const uint32_t hi = a > b ? a : b;
+    if (hi > mx) mx = hi;
   }
+  max_id = mx;
+  return k;
+}
+
+uint64_t count_newlines(const char *p, const char *end) {
+  uint64_t c = 0;
+  while (p < end) {
+    const void *q = std::memchr(p, '\n', (size_t)(end - p));
+    if (!q) break;
+    c++;
+    p = (const char *)q + 1;
+  }
+  return c;
 }
 
 // ---- MatrixMarket coordinate files -----------------------------------------
@@ -274,38 +287,48 @@ int stc_parse_graph(const char *buf, uint64_t len, int threads,
     lo[t] = s;
     hi[t] = e;
   }
-  std::vector<Chunk> parts(nt);
+  // Two passes, both threaded: each chunk's newline count bounds its entries,
+  // so one allocation takes every chunk's slice and the threads parse
+  // straight into the output; then only the chunks after a dropped line
+  // (self-loop, comment, blank) move down.  No per-thread buffers, no serial
+  // copy of the whole edge list.
+  std::vector<uint64_t> base(nt + 1, 0), kept(nt, 0);
+  std::vector<uint32_t> mx(nt, 0);
+  {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t] { base[t + 1] = count_newlines(lo[t], hi[t]) + 1; });
+    for (auto &th : pool) th.join();
+  }
+  for (int t = 0; t < nt; t++) base[t + 1] += base[t];
+  const size_t cap = base[nt] ? base[nt] : 1;
+  g->u = (uint32_t *)std::malloc(cap * 4);
+  g->v = (uint32_t *)std::malloc(cap * 4);
+  g->sign = (int8_t *)std::malloc(cap);
+  if (!g->u || !g->v || !g->sign) {
+    stc_graph_free(g);
+    return STC_E_OOM;
+  }
   {
     std::vector<std::thread> pool;
     for (int t = 0; t < nt; t++)
       pool.emplace_back([&, t] {
-        if (lo[t] < hi[t]) parse_range(lo[t], hi[t], parts[t]);
+        if (lo[t] < hi[t])
+          kept[t] = parse_range(lo[t], hi[t], g->u + base[t], g->v + base[t], g->sign + base[t],
+                                mx[t]);
       });
     for (auto &th : pool) th.join();
   }
   uint64_t m = 0;
   uint32_t max_id = 0;
-  for (auto &c : parts) {
-    m += c.u.size();
-    if (c.max_id > max_id) max_id = c.max_id;
-  }
-  size_t mm = m ? m : 1;
-  g->u = (uint32_t *)std::malloc(mm * 4);
-  g->v = (uint32_t *)std::malloc(mm * 4);
-  g->sign = (int8_t *)std::malloc(mm);
-  if (!g->u || !g->v || !g->sign) {
-    stc_graph_free(g);
-    return STC_E_OOM;
-  }
-  uint64_t off = 0;
-  for (auto &c : parts) {
-    size_t k = c.u.size();
-    if (k) {
-      std::memcpy(g->u + off, c.u.data(), k * 4);
-      std::memcpy(g->v + off, c.v.data(), k * 4);
-      std::memcpy(g->sign + off, c.s.data(), k);
+  for (int t = 0; t < nt; t++) {
+    if (kept[t] && m != base[t]) {  // move the chunk down over the gap
+      std::memmove(g->u + m, g->u + base[t], kept[t] * 4);
+      std::memmove(g->v + m, g->v + base[t], kept[t] * 4);
+      std::memmove(g->sign + m, g->sign + base[t], kept[t]);
     }
-    off += k;
+    m += kept[t];
+    if (mx[t] > max_id) max_id = mx[t];
   }
   g->m = m;
   // Reference: n = max_id + 1 with max_id starting at 0 (graph_io.h:162,210).

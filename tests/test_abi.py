@@ -87,6 +87,34 @@ def test_parser_threads_and_file_roundtrip(tmp_path):
     assert n == g.n and np.array_equal(u, g.u) and np.array_equal(s, g.sign)
 
 
+@pytest.mark.parametrize("eol", ["\n", "\r\n"])
+def test_parser_threads_with_dropped_lines(eol):
+    """Comments, self-loops, blank lines and signless lines spread over every
+    chunk: each thread count must give the oracle's edge list, so the chunks'
+    output slices close up in file order over the dropped lines."""
+    rng = np.random.default_rng(3)
+    lines = []
+    for i in range(20000):
+        r = rng.random()
+        if r < 0.05:
+            lines.append("# comment %d" % i)
+        elif r < 0.10:
+            lines.append("%d %d 1" % (i % 50, i % 50))
+        elif r < 0.12:
+            lines.append("")
+        else:
+            lines.append("%d\t%d %s" % (rng.integers(0, 5000), rng.integers(0, 5000),
+                                        rng.choice(["1", "-1", "+1", ""])))
+    text = eol.join(lines)
+    n, u, v, s = oracle.parse(text)
+    assert len(u) > 15000
+    for threads in (1, 2, 3, 8, 64):
+        g = stc.parse_graph(text, threads)
+        assert g.n == n and g.m == len(u), threads
+        assert np.array_equal(g.u, u) and np.array_equal(g.v, v), threads
+        assert np.array_equal(g.sign, s), threads
+
+
 def test_load_missing_file():
     with pytest.raises(stc.StcError) as ei:
         stc.load_graph("/nonexistent/graph.txt")
