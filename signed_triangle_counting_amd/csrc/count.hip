@@ -551,7 +551,15 @@ __global__ __launch_bounds__(256) void k_vcount(DevCsr g, const Ctrl *ctrl,
   if (tid < kVLists) s_cnt[tid] = 0;
   if (tid == 0) tune = P->tune;
   pull_sources_of(ctrl, s_pull);
-  __syncthreads();
+  // a table of merge buckets only flags no vertex (demotion runs towards
+  // merge): every list is empty and no flag needs reading
+  const uint8_t st = ctrl->table[tid];
+  const bool listed =
+      __syncthreads_or(st == S_HASH || st == S_BITMAP || st == S_PULL) != 0;
+  if (!listed) {
+    if (tid < kVLists) tile_cnt[tid * ntiles + blockIdx.x] = 0u;
+    return;
+  }
   const bool any_pull = any_pull_in(s_pull);
   uint32_t cnt[kVLists] = {0, 0, 0, 0, 0, 0};
   const uint32_t base = blockIdx.x * kVTile;
