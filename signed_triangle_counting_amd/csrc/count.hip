@@ -832,9 +832,12 @@ __device__ __forceinline__ void found_as(bool found, uint32_t x, uint32_t nb, bo
 }
 
 // ---- merge: one lane per edge ----------------------------------------------
+constexpr uint32_t kLaneRegs = 8;  // lists up to this long are merged in registers
 // Two-pointer merge of the part of N+(u) after v with N+(v); per-lane class
-// counters (two-class mode: c[0] balanced, c[1] unbalanced).
-template <bool CLS>
+// counters (two-class mode: c[0] balanced, c[1] unbalanced). REGS adds the
+// in-register path for two short lists (kept out of the persistent kernel,
+// whose other paths it would crowd).
+template <bool CLS, bool REGS = true>
 __device__ __forceinline__ void merge_lane_edge(const DevCsr &g, uint32_t e, uint32_t c[4]) {
   const uint32_t *__restrict__ col = g.col;
   const uint32_t w = col[e];
@@ -843,6 +846,35 @@ __device__ __forceinline__ void merge_lane_edge(const DevCsr &g, uint32_t e, uin
   const uint32_t ea = g.off[a + 1];
   uint32_t pb = g.off[b];
   const uint32_t eb = g.off[b + 1];
+  if (REGS && ea - pa <= kLaneRegs && eb - pb <= kLaneRegs) {
+    // both lists short: load them whole (independent loads, no chain of one
+    // load per merge step) and compare every pair; ranks are unique in a
+    // list, and the two pads differ from each other and from every rank
+    uint32_t A[kLaneRegs], Bv[kLaneRegs];
+#pragma unroll
+    for (uint32_t i = 0; i < kLaneRegs; i++) {
+      A[i] = pa + i < ea ? col[pa + i] : 0xFFFFFFFFu;
+      Bv[i] = pb + i < eb ? col[pb + i] : 0xFFFFFFFDu;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kLaneRegs; i++)
+#pragma unroll
+      for (uint32_t j = 0; j < kLaneRegs; j++) {
+        if ((A[i] >> 1) == (Bv[j] >> 1)) {
+          const uint32_t neg = sab + (A[i] & 1u) + (Bv[j] & 1u);
+          if (CLS) {
+            c[0] += neg == 3;
+            c[1] += neg == 2;
+            c[2] += neg == 1;
+            c[3] += neg == 0;
+          } else {
+            c[0] += (neg & 1u) ^ 1u;
+            c[1] += neg & 1u;
+          }
+        }
+      }
+    return;
+  }
   uint32_t x = col[pa], y = col[pb];
   while (true) {
     const uint32_t xr = x >> 1, yr = y >> 1;
@@ -1909,7 +1941,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
         if (valid) {
           const uint32_t e = lane_list[i];
           gid = graph_of[g.src[e]];
-          if constexpr (!CLS) merge_lane_edge<false>(g, e, c);  // batches: two-class
+          if constexpr (!CLS) merge_lane_edge<false, false>(g, e, c);  // batches: two-class
         }
         lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
       } else if (it < b2) {  // hash vertex
