@@ -202,8 +202,6 @@ __global__ void k_dedupe_flag(const uint64_t *__restrict__ key, uint64_t m,
   }
 }
 
-constexpr int kHubRounds = 4;
-
 __global__ void k_compact_degree(const uint64_t *__restrict__ key,
                                  const uint32_t *__restrict__ flag,
                                  const uint32_t *__restrict__ pos, uint64_t m,
@@ -226,20 +224,17 @@ __global__ void k_compact_degree(const uint64_t *__restrict__ key,
         hi = (uint32_t)((k >> 1) & mask);
       }
     }
-    // upper endpoints are scattered, but a hub is the upper endpoint of many
-    // lanes: a few leader rounds fold each of the commonest values of the wave
-    // into one atomic, and the lanes left add one each
-    uint64_t pending = __ballot(keep);
-#pragma unroll
-    for (int r = 0; r < kHubRounds; r++) {
-      if (!pending) break;
-      const uint32_t hv =
-          (uint32_t)__builtin_amdgcn_readlane((int)hi, (int)__builtin_ctzll(pending));
-      const uint64_t same = __ballot(hi == hv) & pending;
-      if (lane == (uint32_t)__builtin_ctzll(pending)) atomicAdd(&deg[hv], (uint32_t)__popcll(same));
-      pending &= ~same;
+    // upper endpoints are scattered, and a hub is the upper endpoint of many
+    // lanes of every wave: match equal values over the wave bit by bit (one
+    // ballot per id bit), so each distinct value takes one atomic with its
+    // lane count and a hub's counter sees one atomic per wave
+    uint64_t peers = __ballot(keep);
+    for (int b = 0; b < nb; b++) {
+      const bool bit = (hi >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
     }
-    if ((pending >> lane) & 1u) atomicAdd(&deg[hi], 1u);
+    if (keep && (peers & lanemask_lt()) == 0) atomicAdd(&deg[hi], (uint32_t)__popcll(peers));
     // lower endpoints are sorted: one atomic per run of equal lo in the wave
     const uint32_t prev = (uint32_t)__shfl_up((int)lo, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || prev != lo);
