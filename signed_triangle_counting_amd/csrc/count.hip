@@ -331,7 +331,10 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl,
     if (in) {
       const uint32_t a = g.src[e], b = g.col[e] >> 1;
       const uint32_t da = g.off[a + 1] - (uint32_t)e - 1u;
-      const uint32_t cb = g.dcls[b];  // the target's class: one byte per edge gathered
+      // the target's class: one byte gathered per edge that has a suffix (an
+      // edge with da = 0 is skipped whatever its target, and on graphs whose
+      // class array outgrows L2 the gather is a random line fetch)
+      const uint32_t cb = da ? g.dcls[b] : 0xFFu;
       valid = da && cb != 0xFFu;
       if (valid) bk = (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb;
       bk8[e] = valid ? (uint8_t)bk : kNoBucket;
