@@ -113,8 +113,15 @@ typedef struct stc_stats {
                               count with workgroup-pull edges runs both pull
                               slots in one launch, reported in the last slot
                               (wave-pull then reads about 0) */
-  double bytes_kernel[8];  /* algorithmic bytes 4(d+(u)+d+(v)) of the
+  double bytes_kernel[8];  /* merge-model bytes 4(d+(u)+d+(v)) of the
                               oriented edges each of those kernels counted
+                              (the same pull folding) */
+  double alg_bytes_kernel[8]; /* bytes each kernel's strategy must read: the
+                              staged lists once per item, the streamed list
+                              of every counted edge (pull: the 4 da bytes of
+                              the source's suffix; hash and push bitmap: the
+                              4 d+(v) bytes of the target's list; merge: both),
+                              and the metadata of every edge and item visited
                               (the same pull folding) */
 } stc_stats;
 

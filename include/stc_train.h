@@ -67,15 +67,6 @@ int stc_load_models(stc_ctx *ctx, const char *dir);
 /* Apply tune parameters given in the tune.txt text format. */
 int stc_set_tune(stc_ctx *ctx, const char *text);
 
-/* HBM read bandwidth of device `device` for the counting kernels' access
- * patterns, in TB/s: out_tbps[0] a sequential read of a 4 GiB buffer;
- * out_tbps[1..4] reads of random 128-byte-aligned segments of 128, 256, 512
- * and 1024 bytes, 8 segments in flight per wave.  Best of 5 launches of about
- * 4 GB each.  No counterpart in the reference: a measurement hook that gives
- * the pull kernels' scattered suffix reads a ceiling for their own pattern.
- * Allocates and frees 4 GiB of device memory. */
-int stc_probe_read_bandwidth(int device, double out_tbps[5]);
-
 #ifdef __cplusplus
 }
 #endif

@@ -46,7 +46,7 @@ EXPORTED = [
     "stc_gen_er", "stc_gen_grid", "stc_gen_rmat", "stc_gen_hub", "stc_gen_batch",
     "stc_batch_free", "stc_gen_workload", "stc_gen_workload_batch",
     "stc_bucket_histogram", "stc_count_with_table", "stc_bucket_features",
-    "stc_train_dispatcher", "stc_load_models", "stc_set_tune", "stc_probe_read_bandwidth",
+    "stc_train_dispatcher", "stc_load_models", "stc_set_tune",
 ]
 
 TABLE = {"skip": 0, "merge": 1, "hash": 2, "bitmap": 3, "pull": 4}
@@ -99,13 +99,14 @@ class Stats(C.Structure):
                 ("ms_bin", C.c_float), ("edges_merge", C.c_uint64),
                 ("edges_hash", C.c_uint64), ("edges_bitmap", C.c_uint64), ("edges_pull", C.c_uint64),
                 ("edges_skipped", C.c_uint64), ("dispatch_used", C.c_int),
-                ("ms_kernel", C.c_float * 8), ("bytes_kernel", C.c_double * 8)]
+                ("ms_kernel", C.c_float * 8), ("bytes_kernel", C.c_double * 8),
+                ("alg_bytes_kernel", C.c_double * 8)]
 
     def as_dict(self):
         d = {}
         for k, _ in self._fields_:
             v = getattr(self, k)
-            d[k] = list(v) if k in ("ms_kernel", "bytes_kernel") else v
+            d[k] = list(v) if k in ("ms_kernel", "bytes_kernel", "alg_bytes_kernel") else v
         return d
 
 
@@ -167,7 +168,6 @@ def lib():
                                        C.c_float, C.c_uint64, C.c_char_p]
     L.stc_load_models.argtypes = [C.c_void_p, C.c_char_p]
     L.stc_set_tune.argtypes = [C.c_void_p, C.c_char_p]
-    L.stc_probe_read_bandwidth.argtypes = [C.c_int, P(C.c_double)]
     _lib_handle = L
     return L
 
@@ -305,16 +305,6 @@ def gen_batch(k, n_lo=64, n_hi=65536, d_lo=4.0, d_hi=64.0, p_lo=0.05, p_hi=0.5,
     _check(lib().stc_gen_batch(k, n_lo, n_hi, d_lo, d_hi, p_lo, p_hi, seed, threads,
                                C.byref(p)))
     return Batch(p)
-
-
-def probe_read_bandwidth(device=0):
-    """HBM read bandwidth (TB/s) of the device for a sequential read and for
-    random 128-byte-aligned segments of 128, 256, 512 and 1024 bytes
-    (stc_probe_read_bandwidth)."""
-    out = (C.c_double * 5)()
-    _check(lib().stc_probe_read_bandwidth(device, out))
-    keys = ("sequential", "segment_128B", "segment_256B", "segment_512B", "segment_1024B")
-    return {k: round(float(v), 3) for k, v in zip(keys, out)}
 
 
 def gen_workload_batch(k=4096, threads=0):

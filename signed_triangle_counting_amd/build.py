@@ -16,7 +16,9 @@ OBJ = os.path.join(HERE, "_obj")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["csr_build.hip", "count.hip", "mlp.hip", "probe.hip"]
+HIP_SOURCES = ["csr_build.hip", "count.hip", "mlp.hip"]
+# measurement helper for bench.py only, built into its own library
+PROBE_SOURCE = "bench_probe.hip"
 HIP_HOST_SOURCES = ["api.cpp", "upload.cpp"]  # host code that calls the HIP runtime
 HOST_SOURCES = ["gen.cpp", "graph_io.cpp"]
 
@@ -37,7 +39,7 @@ def _run(cmd):
 def _compile(src):
     path = os.path.join(CSRC, src)
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    if src in HIP_SOURCES:
+    if src in HIP_SOURCES or src == PROBE_SOURCE:
         cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", out]
     elif src in HIP_HOST_SOURCES:
         cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", path, "-o", out]
@@ -53,13 +55,17 @@ def build(verbose=False):
         raise RuntimeError("hipcc not found at " + HIPCC)
     os.makedirs(OBJ, exist_ok=True)
     srcs = HIP_SOURCES + HIP_HOST_SOURCES + HOST_SOURCES
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        objs = list(ex.map(_compile, srcs + [PROBE_SOURCE]))
+    probe_obj = objs.pop()
     lib = os.path.join(HERE, "libstc.so")
     tmp = lib + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
          ["-pthread", "-ldl"])
     os.replace(tmp, lib)
+    probe = os.path.join(HERE, "libstc_bench_probe.so")
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", probe + ".tmp", probe_obj])
+    os.replace(probe + ".tmp", probe)
     cli_obj = os.path.join(OBJ, "cli.o")
     _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "cli.cpp"), "-o", cli_obj])
     exe = os.path.join(HERE, "stc")

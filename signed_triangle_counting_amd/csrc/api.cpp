@@ -501,7 +501,7 @@ int stc_get_stats(stc_ctx *c, stc_stats *out) {
     // bucket bytes and strategy table (kept out of the timed counting path).
     (void)hipSetDevice(c->device);
     CountResult res;
-    hipError_t e = enqueue_stats(c->g, c->dp, c->cb, c->num_cus, c->st);
+    hipError_t e = enqueue_stats(c->g, c->dp, c->cb, c->num_cus, c->batch_ready, c->st);
     if (e == hipSuccess) e = read_count(c->cb, false, res, c->st);
     if (e != hipSuccess) return hip_fail(c, e, "count statistics");
     // kernel order: merge-lane, merge-wave, wave-hash, workgroup-hash,
@@ -512,10 +512,15 @@ int stc_get_stats(stc_ctx *c, stc_stats *out) {
     c->stats.edges_hash = res.kern_edges[2] + res.kern_edges[3];
     c->stats.edges_bitmap = res.kern_edges[4] + res.kern_edges[5];
     c->stats.edges_pull = res.kern_edges[6] + res.kern_edges[7];
-    for (int k = 0; k < kKernels; k++) c->stats.bytes_kernel[k] = 4.0 * (double)res.kern_words[k];
+    for (int k = 0; k < kKernels; k++) {
+      c->stats.bytes_kernel[k] = 4.0 * (double)res.kern_words[k];
+      c->stats.alg_bytes_kernel[k] = (double)res.kern_bytes[k];
+    }
     if (c->fused_pull && res.kern_edges[7] > 0) {  // one launch counted both pull slots
       c->stats.bytes_kernel[7] += c->stats.bytes_kernel[6];
       c->stats.bytes_kernel[6] = 0.0;
+      c->stats.alg_bytes_kernel[7] += c->stats.alg_bytes_kernel[6];
+      c->stats.alg_bytes_kernel[6] = 0.0;
     }
     c->kstats_pending = false;
   }

@@ -2,12 +2,12 @@
 // to reads of random short segments, the access pattern of the pull kernels'
 // suffix streams (a few hundred bytes each at scattered offsets).  It gives
 // those kernels' measured traffic a ceiling for their own pattern, beside
-// the data-sheet peak.  Measurement only; nothing in the counting path calls it.
+// the data-sheet peak.  Measurement only: it is built into its own library,
+// libstc_bench_probe.so, which only bench.py --hbm-probe loads; no product
+// header declares it and libstc.so does not contain it.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-
-#include "../../include/stc_train.h"
 
 namespace {
 
@@ -103,13 +103,17 @@ hipError_t probe_segments(const uint32_t *a, uint32_t *out, int grid, hipStream_
 
 }  // namespace
 
-extern "C" int stc_probe_read_bandwidth(int device, double out_tbps[5]) {
-  if (!out_tbps) return STC_E_ARG;
+// out_tbps[0]: sequential read of a 4 GiB buffer; out_tbps[1..4]: random
+// 128-byte-aligned segments of 128, 256, 512 and 1024 bytes, 8 in flight per
+// wave.  Best of 5 launches of about 4 GB each.  Returns 0, -1 (bad
+// argument), -3 (out of memory), -4 (HIP error) or -7 (no such device).
+extern "C" int bench_probe_read_bandwidth(int device, double out_tbps[5]) {
+  if (!out_tbps) return -1;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return STC_E_NODEV;
-  if (hipSetDevice(device) != hipSuccess) return STC_E_HIP;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return -7;
+  if (hipSetDevice(device) != hipSuccess) return -4;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return STC_E_HIP;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return -4;
   const int grid = prop.multiProcessorCount * 8;  // 8 workgroups of 256 threads per CU
   uint32_t *a = nullptr, *out = nullptr;
   hipStream_t st = nullptr;
@@ -138,5 +142,5 @@ extern "C" int stc_probe_read_bandwidth(int device, double out_tbps[5]) {
   if (out) (void)hipFree(out);
   if (a) (void)hipFree(a);
   if (e != hipSuccess) (void)hipGetLastError();
-  return e == hipSuccess ? STC_OK : oom ? STC_E_OOM : STC_E_HIP;
+  return e == hipSuccess ? 0 : oom ? -3 : -4;
 }

@@ -63,6 +63,7 @@ struct Ctrl {
                                      // (launch order); slot 8: left out by
                                      // a partial table, slot 9: skipped
   unsigned long long kern_words[10]; // sum of d+(u) + d+(v) over those edges
+  unsigned long long kern_bytes[10]; // bytes each kernel's strategy reads (see k_edge_stats)
   uint8_t table[kBuckets];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl alignment");
@@ -667,25 +668,38 @@ __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams
 // workgroup-bitmap, wave-pull, workgroup-pull; pull edges by the target
 // list's length); slot 8 the edges a partial table leaves out, slot 9 the
 // edges that cannot close a triangle.
+//   kern_bytes is what each kernel's strategy must read, its algorithmic
+// bytes (the roofline numerator), in two parts.  Per counted edge (u -> v),
+// with da = |N+(u) after v| and db = d+(v), added here:
+//   merge (lane or wave)  24 + 4 (da + db): the list entry, col[e], src[e],
+//                         off[u + 1], off[v], off[v + 1], then both lists
+//                         (the merge walks the suffix of N+(u) and N+(v));
+//   hash, push bitmap     4 db: the streamed target list N+(v);
+//   pull                  4 da: the streamed suffix of N+(u) after v;
+// plus 4 for the graph_of read of a merge edge in a batch.  Per item of the
+// vertex lists, added by k_list_stats: the staged list and the metadata of
+// every edge the item visits.
 __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
                                                     const DispatchParams *P,
-                                                    const uint8_t *__restrict__ bk8) {
+                                                    const uint8_t *__restrict__ bk8,
+                                                    uint32_t batch) {
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
-  __shared__ unsigned long long s_e[10], s_w[10];
+  __shared__ unsigned long long s_e[10], s_w[10], s_b[10];
   const int tid = threadIdx.x;
   table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
-  if (tid < 10) s_e[tid] = s_w[tid] = 0;
+  if (tid < 10) s_e[tid] = s_w[tid] = s_b[tid] = 0;
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < g.m; e += stride) {
     const uint32_t bk = bk8[e];
-    uint32_t slot = 9, words = 0;
+    uint32_t slot = 9, words = 0, bytes = 0;
     if (bk != kNoBucket) {
       const uint32_t a = g.src[e], b = g.col[e] >> 1;
       const uint32_t la = g.off[a], ea = g.off[a + 1];
       const uint32_t db = g.off[b + 1] - g.off[b];
+      const uint32_t da = ea - (uint32_t)e - 1u;
       const uint8_t s = bucket_strategy(
           table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
           [&] { return span_of(g.col, g.off[b], g.off[b + 1]); }, (int)bk);
@@ -697,14 +711,69 @@ __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
              : s == S_BITMAP     ? (small ? 4u : 5u)
                                  : (small_b ? 6u : 7u);
       words = (ea - la) + db;
+      bytes = s == S_SKIP                              ? 0u
+              : (s == S_MERGE_LANE || s == S_MERGE_WAVE) ? 24u + 4u * (da + db) + (batch ? 4u : 0u)
+              : s == S_PULL                            ? 4u * da
+                                                       : 4u * db;
     }
     atomicAdd(&s_e[slot], 1ull);
     atomicAdd(&s_w[slot], (unsigned long long)words);
+    if (bytes) atomicAdd(&s_b[slot], (unsigned long long)bytes);
   }
   __syncthreads();
   if (tid < 10) {
     if (s_e[tid]) atomicAdd(&ctrl->kern_edges[tid], s_e[tid]);
     if (s_w[tid]) atomicAdd(&ctrl->kern_words[tid], s_w[tid]);
+    if (s_b[tid]) atomicAdd(&ctrl->kern_bytes[tid], s_b[tid]);
+  }
+}
+
+// Algorithmic bytes of the vertex-list items (the second part of
+// kern_bytes), one thread per list entry:
+//   push source a (hash, bitmap; workgroup or wave)  4 + 8 + 8 + 4 d + 12 d:
+//       the list entry, off[a], off[a + 1], the list's two ends (its rank
+//       span), the staged list N+(a), and per out-edge j its column word and
+//       the target's two offsets (every out-edge is visited for the strategy
+//       test);
+//   pull chunk of target v with c in-edges  8 + 12 + 8 + 4 d + 8 c: the entry
+//       and its target, off[v], off[v + 1], tin_off[v + 1], the list's two
+//       ends, the staged list N+(v), and one 8-byte in-edge record per
+//       in-edge of the chunk;
+// plus 4 for the graph_of read of each item in a batch.  List L's entries sit
+// at vlist + L vstride (targets of the pull lists 2 strides after them).
+__global__ __launch_bounds__(256) void k_list_stats(DevCsr g, Ctrl *ctrl,
+                                                    const uint32_t *__restrict__ vlist,
+                                                    uint64_t vstride, uint32_t batch) {
+  __shared__ unsigned long long s_b[kVLists];
+  const int tid = threadIdx.x;
+  if (tid < kVLists) s_b[tid] = 0;
+  __syncthreads();
+  const uint32_t extra = batch ? 4u : 0u;
+  for (int L = 0; L < kVLists; L++) {
+    const uint32_t len = ctrl->len[2 + L];
+    unsigned long long acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + tid; i < len; i += gridDim.x * blockDim.x) {
+      if (L < 4) {
+        const uint32_t a = vlist[(uint64_t)L * vstride + i];
+        const uint32_t d = g.off[a + 1] - g.off[a];
+        acc += 20u + 16ull * d + extra;
+      } else {
+        const uint32_t j0 = vlist[(uint64_t)L * vstride + i];
+        const uint32_t v = vlist[(uint64_t)(L + 2) * vstride + i];
+        const uint32_t chunk = L == 4 ? kPullBlockChunk : kPullWaveChunk;
+        const uint32_t c = min(j0 + chunk, g.tin_off[v + 1]) - j0;
+        const uint32_t d = g.off[v + 1] - g.off[v];
+        acc += 28u + 4ull * d + 8ull * c + extra;
+      }
+    }
+    if (acc) atomicAdd(&s_b[L], acc);
+  }
+  __syncthreads();
+  // list -> kernel slot: workgroup hash 3, workgroup bitmap 5, wave hash 2,
+  // wave bitmap 4, workgroup pull 7, wave pull 6
+  if (tid < kVLists && s_b[tid]) {
+    const int slot[kVLists] = {3, 5, 2, 4, 7, 6};
+    atomicAdd(&ctrl->kern_bytes[slot[tid]], s_b[tid]);
   }
 }
 
@@ -1849,12 +1918,20 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 }
 
 // ---- batched counting: one persistent kernel over the union of k graphs ----
-// (A single graph's count runs its pull items through the same kernel as a
-// batch of one: graph_of null, pull_only set, queues q1, q2 that no other
-// kernel of that count uses; two-class totals go to per_graph[0..1], the
-// four-class variant (CLS) adds its classes to ctrl->t.  The two pull phases
-// then share one launch, with no boundary and tail between the workgroup and
-// the wave chunks.  Batches are two-class.)
+// Three instantiations by the item kinds they compile (KIND):
+//   kKindAll        a batch with merge, hash or bitmap items (two-class);
+//   kKindPullBatch  a batch whose only items are pull chunks (the learned
+//                   table's choice for batches of small graphs);
+//   kKindPullOne    one graph's pull items, as a batch of one: graph_of null,
+//                   queues q1, q2 that no other kernel of that count uses;
+//                   two-class totals go to per_graph[0..1], the four-class
+//                   variant (CLS) adds its classes to ctrl->t.  The two pull
+//                   phases then share one launch, with no boundary and tail
+//                   between the workgroup and the wave chunks.
+// The pull-only instantiations carry no merge, hash or push-bitmap state, so
+// they fit the 64-VGPR cap without spilling to scratch (the all-kinds kernel
+// spills 36 bytes per lane).  Both batch instantiations are launched; each
+// reads the list lengths and the one whose kinds are absent returns at once.
 // Two phases per workgroup, each draining one queue:
 //   1. workgroup items, popped by the whole workgroup (heaviest kinds first):
 //      merge-lane chunks of kLaneItem edges (one per thread), hash vertices,
@@ -1863,7 +1940,7 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 //      merge-wave edges, wave-hash and wave-bitmap vertices, wave-pull
 //      in-edge chunks.  A wave never waits for the others, so short and long
 //      items mix without idling the workgroup.
-// Both queues keep contiguous per-XCD ranges: a batch graph's ranks are
+// A batch's queues keep contiguous per-XCD ranges: a batch graph's ranks are
 // contiguous, so each small graph stays in one XCD's L2 (dealing the wave
 // pops over the XCDs instead made bitmap counting of the 4,096-graph batch
 // 109 -> 168 ms).  The workgroup is capped at 64 VGPRs (8 waves per SIMD).
@@ -1882,6 +1959,7 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 constexpr uint32_t kBatchBlock = 512;
 constexpr uint32_t kLaneItem = kBatchBlock;
 constexpr uint32_t kWavesPerBatchBlock = kBatchBlock / 64;
+constexpr int kKindAll = 0, kKindPullBatch = 1, kKindPullOne = 2;
 
 __device__ __forceinline__ void lane_flush_graph(unsigned long long *per_graph, bool valid,
                                                  uint32_t gid, uint32_t c0, uint32_t c1) {
@@ -1902,28 +1980,40 @@ __device__ __forceinline__ void lane_flush_graph(unsigned long long *per_graph, 
   }
 }
 
-template <bool CLS>
+template <bool CLS, int KIND>
 __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
     DevCsr g, Ctrl *ctrl, const DispatchParams *P, const uint32_t *__restrict__ lane_list,
     const uint32_t *__restrict__ wave_list, const uint32_t *__restrict__ vlist, uint64_t vstride,
     const uint32_t *__restrict__ graph_of,
     unsigned long long *__restrict__ per_graph, uint32_t bitmap_words, uint32_t hash_slots,
-    uint32_t window_words, uint32_t region_words, uint32_t q1, uint32_t q2,
-    uint32_t pull_only) {
+    uint32_t window_words, uint32_t region_words, uint32_t q1, uint32_t q2) {
+  // four-class counting serves one graph's pull items only: an all-kinds
+  // four-class kernel would have to count merge-lane items in four classes
+  static_assert(!CLS || KIND == kKindPullOne, "four-class counting is one graph's pull items");
+  constexpr bool ALL = KIND == kKindAll;
   extern __shared__ uint32_t R[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
   const uint32_t tid = threadIdx.x, lane = __lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t keep = pull_only ? 0u : 0xFFFFFFFFu;
-  const uint32_t nl = ctrl->len[0] & keep, nwv = ctrl->len[1] & keep, nh = ctrl->len[2] & keep,
-                 nb = ctrl->len[3] & keep, nhw = ctrl->len[4] & keep, nbw = ctrl->len[5] & keep,
+  const uint32_t others = ctrl->len[0] | ctrl->len[1] | ctrl->len[2] | ctrl->len[3] |
+                          ctrl->len[4] | ctrl->len[5];
+  const uint32_t nl = ALL ? ctrl->len[0] : 0u, nwv = ALL ? ctrl->len[1] : 0u,
+                 nh = ALL ? ctrl->len[2] : 0u, nb = ALL ? ctrl->len[3] : 0u,
+                 nhw = ALL ? ctrl->len[4] : 0u, nbw = ALL ? ctrl->len[5] : 0u,
                  np = ctrl->len[6], nwp = ctrl->len[7];
-  if ((nl | nwv | nh | nb | nhw | nbw | np | nwp) == 0u) return;  // the whole grid: no items
-  // one graph's pull items: without workgroup chunks the wave-pull kernel,
-  // launched before, has counted the wave chunks
-  if (pull_only && np == 0u) return;
+  // the whole grid returns when its kinds are absent: a batch runs the
+  // all-kinds kernel when it has items other than pull chunks, else the
+  // pull-only one; one graph's launch needs workgroup-pull chunks (without
+  // them the wave-pull kernel, launched before, has counted the wave chunks)
+  if constexpr (KIND == kKindAll) {
+    if (others == 0u) return;
+  } else if constexpr (KIND == kKindPullBatch) {
+    if (others != 0u || (np | nwp) == 0u) return;
+  } else {
+    if (np == 0u) return;
+  }
   if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
@@ -1936,7 +2026,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
     QueueCursor cur{blockIdx.x & 7u, 0u};
     uint32_t it;
     while (block_pop(ctrl->queue[q1], total, cur, &s_first, &s_count, it)) {
-      if (it < b1) {  // merge-lane chunk
+      if (ALL && it < b1) {  // merge-lane chunk
         const uint32_t i = it * kLaneItem + tid;
         const bool valid = i < nl;
         uint32_t c[4] = {0, 0, 0, 0};
@@ -1944,15 +2034,15 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
         if (valid) {
           const uint32_t e = lane_list[i];
           gid = graph_of[g.src[e]];
-          if constexpr (!CLS) merge_lane_edge<false, false>(g, e, c);  // batches: two-class
+          merge_lane_edge<false, false>(g, e, c);  // batches: two-class
         }
         lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
-      } else if (it < b2) {  // hash vertex
+      } else if (ALL && it < b2) {  // hash vertex
         const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[it - b1]);
         ga.to(per_graph, graph_of[a]);
         hash_vertex_block<CLS>(g, a, R, table, tune, acc);
         for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
-      } else if (it < b3) {  // bitmap vertex
+      } else if (ALL && it < b3) {  // bitmap vertex
         const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[vstride + it - b2]);
         ga.to(per_graph, graph_of[a]);
         bitmap_vertex_block<CLS>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
@@ -1980,10 +2070,8 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
       // a batch keeps each XCD on a contiguous range (its graphs stay in one
       // L2); one graph's pops are dealt round-robin over the XCDs, which
       // spreads the heavy top of the list (as in the single-graph wave kernels)
-      if (lane == 0 && !(pull_only ? queue_pop<true>(ctrl->queue[q2], total, kWaveChunk, cur,
-                                                     first, count)
-                                   : queue_pop<false>(ctrl->queue[q2], total, kWaveChunk, cur,
-                                                      first, count)))
+      if (lane == 0 && !queue_pop<KIND == kKindPullOne>(ctrl->queue[q2], total, kWaveChunk, cur,
+                                                         first, count))
         count = 0;
       first = __shfl(first, 0, 64);
       count = __shfl(count, 0, 64);
@@ -1995,17 +2083,17 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
                                     first + skip - w3, skip, count)
                        : 0u;
       for (uint32_t it = first; it < first + count; it++) {
-        if (it < w1) {  // merge-wave edge
+        if (ALL && it < w1) {  // merge-wave edge
           const uint32_t e = __builtin_amdgcn_readfirstlane(wave_list[it]);
           ga.to(per_graph, graph_of[g.src[e]]);
           merge_wave_edge<CLS>(g, e, acc);
-        } else if (it < w2) {  // wave-hash vertex
+        } else if (ALL && it < w2) {  // wave-hash vertex
           const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[2 * vstride + it - w1]);
           ga.to(per_graph, graph_of[a]);
           hash_vertex_wave<CLS>(g, a, W, table, tune, acc);
 #pragma unroll
           for (uint32_t k = lane; k < kWaveCap; k += 64) W[k] = 0u;
-        } else if (it < w3) {  // wave-bitmap vertex
+        } else if (ALL && it < w3) {  // wave-bitmap vertex
           const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[3 * vstride + it - w2]);
           ga.to(per_graph, graph_of[a]);
           bitmap_vertex_wave<CLS>(g, a, W, LW, window, table, tune, acc);
@@ -2075,7 +2163,10 @@ hipError_t init_kernel_attributes() {
                        (const void *)k_bitmap_wave<true, false>, (const void *)k_bitmap_wave<false, false>,
                        (const void *)k_bitmap_wave<true, true>,  (const void *)k_bitmap_wave<false, true>,
                        (const void *)k_hash<true>,               (const void *)k_hash<false>,
-                       (const void *)k_two_phase<true>,          (const void *)k_two_phase<false>};
+                       (const void *)k_two_phase<true, kKindPullOne>,
+                       (const void *)k_two_phase<false, kKindPullOne>,
+                       (const void *)k_two_phase<false, kKindAll>,
+                       (const void *)k_two_phase<false, kKindPullBatch>};
   for (const void *f : fns)
     STC_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   done = true;
@@ -2095,45 +2186,72 @@ struct BinOut {
 // between runs; cb.vlist the six vertex lists (vlist_stride words each) and the
 // per-tile list counts and offsets.  Every buffer is sized before the first launch, so a second call
 // with the same graph allocates nothing (and can be captured in a graph).
+// Sizes of the binning pass's buffers for a graph of n vertices and m
+// oriented edges, and where its lists sit inside them.
+struct BinLayout {
+  int bin_grid;
+  uint32_t seg, ntiles;
+  uint64_t seg_words, vstride;
+  size_t lists_bytes, blk_bytes, bk8_bytes, vflag_bytes, vlist_bytes;
+};
+
+BinLayout bin_layout(uint32_t n, uint64_t m, int num_cus) {
+  BinLayout b;
+  b.bin_grid = grid_for(m, 256, num_cus * 8);
+  // per-workgroup segment size: the most edges one workgroup visits
+  const uint64_t stride = (uint64_t)b.bin_grid * 256;
+  b.seg = (uint32_t)(((m + stride - 1) / stride) * 256);
+  b.seg_words = (uint64_t)b.seg * b.bin_grid;
+  b.ntiles = (uint32_t)(((uint64_t)n + kVTile - 1) / kVTile);
+  b.vstride = vlist_stride(n, m);
+  b.lists_bytes = (2 * b.seg_words + 2 * (m ? m : 1)) * 4;
+  b.blk_bytes = (4 * (uint64_t)b.bin_grid + 8) * 4;
+  b.bk8_bytes = m ? m : 1;
+  b.vflag_bytes = ((uint64_t)n + 1) * 4;
+  b.vlist_bytes = ((kVLists + 2) * b.vstride + 2 * kVLists * (uint64_t)b.ntiles + 8) * 4;
+  return b;
+}
+
+void bin_lists(const BinLayout &b, uint64_t m, uint32_t *lists, uint32_t *vlist, BinOut &out) {
+  out.lane_list = lists + 2 * b.seg_words;
+  out.wave_list = out.lane_list + (m ? m : 1);
+  out.vh = vlist;
+  out.vstride = b.vstride;
+  out.vb = out.vh + b.vstride;
+  out.vw = out.vb + b.vstride;
+  out.vwb = out.vw + b.vstride;
+  out.vp = out.vwb + b.vstride;
+  out.vwp = out.vp + b.vstride;
+  out.vpt = out.vwp + b.vstride;  // list L's targets sit 2 strides after it
+  out.vwpt = out.vpt + b.vstride;
+}
+
 hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
                            int num_cus, hipStream_t st, const hipEvent_t *marks, BinOut &out) {
   const uint64_t m = g.m;
   const uint32_t n = g.n;
+  const BinLayout lay = bin_layout(n, m, num_cus);
+  const int bin_grid = lay.bin_grid;
+  const uint32_t seg = lay.seg, ntiles = lay.ntiles;
+  const uint64_t seg_words = lay.seg_words, vstride = lay.vstride;
   STC_TRY(cb.gcount.ensure(sizeof(Ctrl)));
-  const int bin_grid = grid_for(m, 256, num_cus * 8);
-  // per-workgroup segment size: the most edges one workgroup visits
-  const uint64_t stride = (uint64_t)bin_grid * 256;
-  const uint32_t seg = (uint32_t)(((m + stride - 1) / stride) * 256);
-  const uint64_t seg_words = (uint64_t)seg * bin_grid;
-  STC_TRY(cb.lists.ensure((2 * seg_words + 2 * (m ? m : 1)) * 4));
-  STC_TRY(cb.blk.ensure((4 * (uint64_t)bin_grid + 8) * 4));
-  STC_TRY(cb.bk8.ensure(m ? m : 1));
-  if (cb.vflag.bytes < ((uint64_t)n + 1) * 4) {
-    STC_TRY(cb.vflag.ensure(((uint64_t)n + 1) * 4));
+  STC_TRY(cb.lists.ensure(lay.lists_bytes));
+  STC_TRY(cb.blk.ensure(lay.blk_bytes));
+  STC_TRY(cb.bk8.ensure(lay.bk8_bytes));
+  if (cb.vflag.bytes < lay.vflag_bytes) {
+    STC_TRY(cb.vflag.ensure(lay.vflag_bytes));
     hipLaunchKernelGGL(k_fill_words, dim3(grid_for((uint64_t)n + 1, 256, 8192)), dim3(256),
                        0, st, cb.vflag.as<uint32_t>(), cb.vflag.bytes / 4, 0u);
     STC_TRY(hipGetLastError());
   }
-  const uint32_t ntiles = (uint32_t)(((uint64_t)n + kVTile - 1) / kVTile);
-  const uint64_t vstride = vlist_stride(n, m);
-  STC_TRY(cb.vlist.ensure(((kVLists + 2) * vstride + 2 * kVLists * (uint64_t)ntiles + 8) * 4));
+  STC_TRY(cb.vlist.ensure(lay.vlist_bytes));
   Ctrl *ctrl = cb.gcount.as<Ctrl>();
   uint32_t *lane_seg = cb.lists.as<uint32_t>();
   uint32_t *wave_seg = lane_seg + seg_words;
-  out.lane_list = wave_seg + seg_words;
-  out.wave_list = out.lane_list + (m ? m : 1);
+  bin_lists(lay, m, cb.lists.as<uint32_t>(), cb.vlist.as<uint32_t>(), out);
   uint32_t *blk_cnt = cb.blk.as<uint32_t>();
   uint32_t *blk_pos = blk_cnt + 2 * bin_grid;
   uint32_t *vf = cb.vflag.as<uint32_t>();
-  out.vh = cb.vlist.as<uint32_t>();
-  out.vstride = vstride;
-  out.vb = out.vh + vstride;
-  out.vw = out.vb + vstride;
-  out.vwb = out.vw + vstride;
-  out.vp = out.vwb + vstride;
-  out.vwp = out.vp + vstride;
-  out.vpt = out.vwp + vstride;  // list L's targets sit 2 strides after it
-  out.vwpt = out.vpt + vstride;
   uint32_t *tile_cnt = out.vwpt + vstride;
   uint32_t *tile_pos = tile_cnt + kVLists * (uint64_t)ntiles;
 
@@ -2169,7 +2287,7 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
 hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                             CountBuffers &cb, const BinOut &L, const uint32_t *graph_of,
                             unsigned long long *per_graph, int num_cus, hipStream_t st,
-                            uint32_t q1, uint32_t q2, bool pull_only, bool classes);
+                            uint32_t q1, uint32_t q2, bool one_graph, bool classes);
 
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
@@ -2246,9 +2364,16 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
 }
 
 hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
-                         int num_cus, hipStream_t st) {
+                         int num_cus, bool batch, hipStream_t st) {
   hipLaunchKernelGGL(k_edge_stats, dim3(grid_for(g.m, 256, num_cus * 8)), dim3(256), 0, st, g,
-                     cb.gcount.as<Ctrl>(), dp, cb.bk8.as<uint8_t>());
+                     cb.gcount.as<Ctrl>(), dp, cb.bk8.as<uint8_t>(), batch ? 1u : 0u);
+  STC_TRY(hipGetLastError());
+  // the vertex lists of the run just counted (the buffers binning filled)
+  BinOut L;
+  bin_lists(bin_layout(g.n, g.m, num_cus), g.m, cb.lists.as<uint32_t>(), cb.vlist.as<uint32_t>(),
+            L);
+  hipLaunchKernelGGL(k_list_stats, dim3(num_cus * 4), dim3(256), 0, st, g, cb.gcount.as<Ctrl>(),
+                     L.vh, L.vstride, batch ? 1u : 0u);
   return hipGetLastError();
 }
 
@@ -2275,19 +2400,27 @@ size_t batch_lds_bytes(const TuneParams &t) { return 4u * batch_region_words(t);
 hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                             CountBuffers &cb, const BinOut &L, const uint32_t *graph_of,
                             unsigned long long *per_graph, int num_cus, hipStream_t st,
-                            uint32_t q1, uint32_t q2, bool pull_only, bool classes) {
+                            uint32_t q1, uint32_t q2, bool one_graph, bool classes) {
+  // four-class counting is one graph's (its pull items); batches are two-class
+  if (classes && !one_graph) return hipErrorInvalidValue;
   const size_t lds = batch_lds_bytes(hp.tune);
   // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
   const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
-  // the four-class variant counts one graph's pull items only
-  auto kern = classes ? k_two_phase<true> : k_two_phase<false>;
-  hipLaunchKernelGGL(kern, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
-                     cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
-                     graph_of, per_graph,
-                     (1u << hp.tune.bitmap_span_log2) / 16u,
-                     4u << hp.tune.hash_max_log2, (1u << hp.tune.wave_window_log2) / 16u,
-                     (uint32_t)batch_region_words(hp.tune), q1, q2, pull_only ? 1u : 0u);
-  return hipGetLastError();
+  auto launch = [&](auto kern) -> hipError_t {
+    hipLaunchKernelGGL(kern, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
+                       cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
+                       graph_of, per_graph, (1u << hp.tune.bitmap_span_log2) / 16u,
+                       4u << hp.tune.hash_max_log2, (1u << hp.tune.wave_window_log2) / 16u,
+                       (uint32_t)batch_region_words(hp.tune), q1, q2);
+    return hipGetLastError();
+  };
+  if (one_graph)
+    return classes ? launch(k_two_phase<true, kKindPullOne>)
+                   : launch(k_two_phase<false, kKindPullOne>);
+  // a batch: the all-kinds kernel when it has merge, hash or bitmap items,
+  // else the pull-only kernel (each returns at once when it is not the one)
+  STC_TRY(launch(k_two_phase<false, kKindAll>));
+  return launch(k_two_phase<false, kKindPullBatch>);
 }
 
 hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
@@ -2312,6 +2445,7 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_
   for (int k = 0; k < 10; k++) {
     r.kern_edges[k] = h.kern_edges[k];
     r.kern_words[k] = h.kern_words[k];
+    r.kern_bytes[k] = h.kern_bytes[k];
   }
   for (int k = 0; k < kBuckets; k++) {
     r.table[k] = h.table[k];

@@ -219,6 +219,9 @@ struct CountResult {
   // their sum of d+(u) + d+(v)
   uint64_t kern_edges[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t kern_words[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // bytes each kernel's strategy reads: the staged lists, the streamed lists
+  // and the metadata of the edges and items it visits
+  uint64_t kern_bytes[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint8_t table[kBuckets];
   uint32_t bucket_cnt[kBuckets];
 };
@@ -236,7 +239,7 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
 // Per-kernel edge and word counts of the run just enqueued (not timed: enqueue
 // after the run's closing event).
 hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
-                         int num_cus, hipStream_t st);
+                         int num_cus, bool batch, hipStream_t st);
 // Per-vertex counts: pv_rank (4 n, by rank) to out (4 n, by vertex id) through
 // perm (rank -> vertex id).
 hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *perm, uint32_t n,

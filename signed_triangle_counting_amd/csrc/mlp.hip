@@ -2,8 +2,8 @@
 //
 // No reference counterpart (the reference dispatches with a fixed 4x skew
 // rule, Signed_TC/GPU Implementations/GlobalMEM/signed_tc.hxx:48,65).  The
-// model maps a bucket's features to logits over {merge, hash, bitmap}; the
-// counter picks the argmax, so inference needs no exponential.  Training
+// model maps a bucket's features to logits over {merge, hash, bitmap, pull};
+// the counter picks the argmax, so inference needs no exponential.  Training
 // minimises, over samples i, w_i * sum_s softmax(z_i)_s * r_is where r_is is
 // strategy s's cost normalised to [0, 1] within the sample and w_i the time at
 // stake (max - min cost).  One wave64 runs full-batch Adam: each lane

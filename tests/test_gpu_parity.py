@@ -364,8 +364,10 @@ def test_mixed_strategy_tables(ctx, seed):
 
 def test_kernel_statistics_account_for_every_edge(ctx):
     """Per-kernel edge counts (from the counting kernels) plus skipped edges
-    (from binning) add up to the oriented edge count in every mode, and the
-    per-kernel algorithmic bytes add up to the B_alg of the counted edges."""
+    (from binning) add up to the oriented edge count in every mode, the
+    per-kernel merge-model bytes add up to the B_alg of the counted edges, and
+    the strategy bytes of forced merge, bitmap and pull equal the byte model
+    restated from the oracle's CSR mapped into rank space."""
     g = stc.gen_rmat(17, 16, 0.57, 0.19, 0.19, 0.25, 4242)
     o = oracle.Csr(g.n, g.u, g.v, g.sign)
     off, E, _, R = o.arrays()
@@ -380,16 +382,39 @@ def test_kernel_statistics_account_for_every_edge(ctx):
     da = fdr[srcr] - (np.arange(len(key)) - offr[srcr]) - 1  # entries after the target
     counted = (da > 0) & (fdr[dstr] > 0)
     words = int((fdr[srcr] + fdr[dstr])[counted].sum())
+    db = fdr[dstr]
+    # every list fits the LDS structures (no demotion): the byte model below
+    # needs no fit test
+    assert fdr.max() <= 1024
+    # strategy bytes (the statistics' algorithmic bytes, restated):
+    # merge: 24 + 4 (da + db) per counted edge
+    want_merge = int((24 + 4 * (da + db))[counted].sum())
+    # push bitmap: 4 db per counted edge; per source with a counted edge
+    # 20 + 16 d+(u)
+    srcs = np.unique(srcr[counted])
+    want_bitmap = int((4 * db)[counted].sum()) + int((20 + 16 * fdr[srcs]).sum())
+    # pull: 4 da per counted edge; per target with a counted in-edge, chunks
+    # of 2048 (d+ > 64) or 256 in-edges, each 28 + 4 d+(v), and 8 bytes per
+    # in-edge record
+    tg = np.unique(dstr[counted])
+    indeg = np.bincount(dstr, minlength=n)[tg]
+    chunk = np.where(fdr[tg] > 64, 2048, 256)
+    nchunks = (indeg + chunk - 1) // chunk
+    want_pull = (int((4 * da)[counted].sum()) + int((nchunks * (28 + 4 * fdr[tg])).sum()) +
+                 int((8 * indeg).sum()))
+    want_alg = {"merge": want_merge, "bitmap": want_bitmap, "pull": want_pull}
     ctx.prepare(g)
     for mode in MODES:
         ctx.count_prepared(dispatch=mode)
         st = ctx.stats()
         say("stats", mode, st["edges_merge"], st["edges_hash"], st["edges_bitmap"],
-            st["edges_pull"], st["edges_skipped"])
+            st["edges_pull"], st["edges_skipped"], sum(st["alg_bytes_kernel"]))
         assert (st["edges_merge"] + st["edges_hash"] + st["edges_bitmap"] + st["edges_pull"] +
                 st["edges_skipped"]) == st["m_oriented"] == o.m
         assert st["edges_skipped"] == int((~counted).sum())
         assert sum(st["bytes_kernel"]) == 4 * words
+        if mode in want_alg:
+            assert sum(st["alg_bytes_kernel"]) == want_alg[mode], mode
 
 
 def test_batch_two_phase_and_training_hooks(ctx):
