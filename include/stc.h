@@ -150,6 +150,20 @@ int stc_last_classes(const stc_ctx *ctx, stc_classes *out);
 int stc_prepare(stc_ctx *ctx, const stc_edges *g, int g_is_device);
 int stc_count_prepared(stc_ctx *ctx, const stc_opts *opts, stc_counts *out);
 
+/* stc_count_prepared without any device allocation, for callers that cannot
+ * allocate device memory (code under HIP stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * counting pass's buffers (control block, bucket bytes, edge and vertex
+ * lists, scan levels) come from one static device array of 128 MiB that the
+ * library holds on every GPU it runs on until the process ends; that is
+ * enough for graphs of about 4 M oriented edges (the 960 x 960 grid needs
+ * 75 MB).  A larger graph returns STC_E_TOO_LARGE.  Same counts as
+ * stc_count_prepared in every mode, classes included (stc_last_classes);
+ * per-kernel edge and byte statistics read zero.  Every call on one GPU
+ * shares the array and keeps what the last call left there: calls that
+ * overlap on one GPU (two host threads, two contexts) race. */
+int stc_count_prepared_static(stc_ctx *ctx, const stc_opts *opts, stc_counts *out);
+
 /* Per-vertex four-class counts of the prepared graph (stc_prepare): cnt has
  * 4 n entries, and cnt[4 v + k] counts the T_k triangles (k = 0..3: t0..t3 of
  * stc_classes) whose lowest-ranked vertex in the (degree, id) orientation is

@@ -40,7 +40,7 @@ KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitm
 EXPORTED = [
     "stc_create", "stc_destroy", "stc_last_error", "stc_model_status",
     "stc_count", "stc_count_device", "stc_last_classes", "stc_prepare",
-    "stc_count_prepared", "stc_count_batch", "stc_prepare_batch",
+    "stc_count_prepared", "stc_count_prepared_static", "stc_count_batch", "stc_prepare_batch",
     "stc_count_batch_prepared", "stc_count_per_vertex", "stc_get_stats", "stc_run",
     "stc_load_graph", "stc_parse_graph", "stc_write_graph", "stc_graph_free",
     "stc_gen_er", "stc_gen_grid", "stc_gen_rmat", "stc_gen_hub", "stc_gen_batch",
@@ -133,6 +133,7 @@ def lib():
     L.stc_count_device.argtypes = [C.c_void_p, P(Edges), P(Opts), P(Counts)]
     L.stc_prepare.argtypes = [C.c_void_p, P(Edges), C.c_int]
     L.stc_count_prepared.argtypes = [C.c_void_p, P(Opts), P(Counts)]
+    L.stc_count_prepared_static.argtypes = [C.c_void_p, P(Opts), P(Counts)]
     L.stc_count_batch.argtypes = [C.c_void_p, P(Edges), C.c_uint32, P(Opts), P(Counts)]
     L.stc_prepare_batch.argtypes = [C.c_void_p, P(Edges), C.c_uint32]
     L.stc_count_per_vertex.argtypes = [C.c_void_p, P(Opts), C.c_void_p, P(Counts)]
@@ -382,6 +383,14 @@ class Counter:
         out = Counts()
         _check(lib().stc_count_prepared(self._h, C.byref(self._opts(dispatch, repeat, classes)),
                                         C.byref(out)), self._h)
+        return int(out.balanced), int(out.unbalanced)
+
+    def count_prepared_static(self, dispatch="learned", repeat=1, classes=False):
+        """stc_count_prepared_static: the same count with the counting buffers
+        in the library's static device array (no device allocation)."""
+        out = Counts()
+        _check(lib().stc_count_prepared_static(
+            self._h, C.byref(self._opts(dispatch, repeat, classes)), C.byref(out)), self._h)
         return int(out.balanced), int(out.unbalanced)
 
     def count_per_vertex(self, dispatch="learned", repeat=1):

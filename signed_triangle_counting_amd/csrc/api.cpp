@@ -49,6 +49,11 @@ struct stc_ctx {
   UploadRing ring;  // pinned staging for host edge lists (allocated on first use)
   bool fused_pull = false;  // last count was one graph's: with workgroup-pull edges, one
                             // launch counted both pull slots (its time in the last)
+  // Device address of the library's static scratch array on this context's
+  // device (hipGetSymbolAddress at stc_create); null with scratch_err set when
+  // that call failed.
+  void *scratch = nullptr;
+  hipError_t scratch_err = hipSuccess;
 };
 
 namespace {
@@ -297,6 +302,15 @@ int stc_create(int device, const char *model_dir, stc_ctx **out) {
   for (int i = 0; i < kMarks && e == hipSuccess; i++) e = hipEventCreate(&c->marks[i]);
   if (e == hipSuccess) e = hipMalloc(&c->dp, sizeof(DispatchParams));
   if (e == hipSuccess) e = init_kernel_attributes();
+  if (e == hipSuccess) {
+    // the static scratch's address on this device (the allocation-free count
+    // reports the error if this fails; nothing else needs it)
+    c->scratch_err = static_scratch_address(&c->scratch);
+    if (c->scratch_err != hipSuccess) {
+      c->scratch = nullptr;
+      (void)hipGetLastError();
+    }
+  }
   if (e != hipSuccess) {
     int rc = hip_fail(c, e, "stc_create");
     stc_destroy(c);
@@ -408,10 +422,14 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
 
 }  // extern "C"
 
+// Binning + counting of the prepared graph.  cbuf: the buffers to count
+// with (null: the context's own, grown on first use; else borrowed views of
+// the static scratch, for which no per-kernel statistics are kept).
 static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bool classes,
-                      stc_counts *out, CountResult *res_out) {
+                      stc_counts *out, CountResult *res_out, CountBuffers *cbuf = nullptr) {
   if (!c->prepared) return fail(c, STC_E_ARG, "no prepared graph");
   (void)hipSetDevice(c->device);
+  CountBuffers &cb = cbuf ? *cbuf : c->cb;
   c->hp.mode = mode;
   c->hp.tune.allow_pull = 1;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
@@ -422,7 +440,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   for (int r = 0; r < repeat; r++) {
     e = hipEventRecord(c->ev[0], c->st);
     if (e == hipSuccess)
-      e = enqueue_count(c->g, c->hp, c->dp, c->cb, classes, c->num_cus, c->st, c->marks);
+      e = enqueue_count(c->g, c->hp, c->dp, cb, classes, c->num_cus, c->st, c->marks);
     if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
     if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
     float ms = 0.f, msb = 0.f;
@@ -438,7 +456,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
     bins.push_back(msb);
   }
   CountResult res;
-  e = read_count(c->cb, classes, res, c->st);
+  e = read_count(cb, classes, res, c->st);
   if (e != hipSuccess) return hip_fail(c, e, "count readback");
   if (classes) {
     c->classes = {res.t[0], res.t[1], res.t[2], res.t[3]};
@@ -455,7 +473,14 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
   c->stats.ms_bin = bins.back();
   for (int k = 0; k < kKernels; k++) c->stats.ms_kernel[k] = median(kt[k]);
-  c->kstats_pending = true;
+  // per-kernel statistics are computed later from the binning buffers, which
+  // a static-scratch count does not keep: its per-kernel sizes read zero
+  c->kstats_pending = cbuf == nullptr;
+  if (cbuf) {
+    c->stats.edges_merge = c->stats.edges_hash = c->stats.edges_bitmap = 0;
+    c->stats.edges_pull = c->stats.edges_skipped = 0;
+    for (int k = 0; k < kKernels; k++) c->stats.bytes_kernel[k] = c->stats.alg_bytes_kernel[k] = 0;
+  }
   c->stats.dispatch_used = (mode == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : mode;
   if (res_out) *res_out = res;
   return STC_OK;
@@ -471,6 +496,30 @@ int stc_count_prepared(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
   if (dispatch < STC_LEARNED || dispatch > STC_FORCE_PULL)
     return fail(c, STC_E_ARG, "bad dispatch");
   return count_core(c, dispatch, nullptr, repeat, classes, out, nullptr);
+}
+
+int stc_count_prepared_static(stc_ctx *c, const stc_opts *opts, stc_counts *out) {
+  if (!c || !out) return STC_E_ARG;
+  int dispatch = opts ? opts->dispatch : STC_LEARNED;
+  int repeat = opts && opts->repeat > 0 ? opts->repeat : 1;
+  bool classes = opts && opts->classes;
+  if (dispatch < STC_LEARNED || dispatch > STC_FORCE_PULL)
+    return fail(c, STC_E_ARG, "bad dispatch");
+  if (!c->prepared) return fail(c, STC_E_ARG, "no prepared graph");
+  if (!c->scratch)
+    return hip_fail(c, c->scratch_err != hipSuccess ? c->scratch_err : hipErrorInvalidValue,
+                    "static scratch address");
+  (void)hipSetDevice(c->device);
+  CountBuffers sb;
+  size_t need = 0;
+  if (!carve_count_buffers(c->scratch, kStaticScratchBytes, c->g.n, c->g.m, c->num_cus, sb, &need))
+    return fail(c, STC_E_TOO_LARGE,
+                "graph needs " + std::to_string(need) + " bytes of counting scratch; the static "
+                "scratch holds " + std::to_string(kStaticScratchBytes) + " (use stc_count_prepared)");
+  // the scratch keeps the last call's contents: the vertex flags must start at zero
+  hipError_t e = clear_vflags(sb, c->g.n, c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "static scratch clear");
+  return count_core(c, dispatch, nullptr, repeat, classes, out, nullptr, &sb);
 }
 
 int stc_count(stc_ctx *c, const stc_edges *g, const stc_opts *opts, stc_counts *out) {

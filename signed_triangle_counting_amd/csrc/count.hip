@@ -2289,6 +2289,53 @@ hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const Dis
                             unsigned long long *per_graph, int num_cus, hipStream_t st,
                             uint32_t q1, uint32_t q2, bool one_graph, bool classes);
 
+// ---- static scratch (the allocation-free counting entry point) -------------
+// The counting pass's buffers for callers that cannot allocate device memory
+// (code under stream capture, serving loops): a file-scope array that every
+// GPU running this library holds until the process ends.  In host code its
+// name is only a stand-in; the device address comes from
+// static_scratch_address, once per device with that device current, and only
+// that address is handed to kernels and fills.  Every call on one GPU shares
+// the array, so calls that overlap on one GPU race.
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+hipError_t static_scratch_address(void **ptr) {
+  *ptr = nullptr;
+  return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+bool carve_count_buffers(void *base, size_t bytes, uint32_t n, uint64_t m, int num_cus,
+                         CountBuffers &cb, size_t *need) {
+  const BinLayout lay = bin_layout(n, m, num_cus);
+  const uint64_t scan_n[2] = {2 * (uint64_t)lay.bin_grid, (uint64_t)kVLists * lay.ntiles};
+  size_t level[4];
+  for (int l = 0; l < 4; l++)
+    level[l] = std::max<size_t>(256, std::max(scan_level_bytes(scan_n[0], l),
+                                              scan_level_bytes(scan_n[1], l)));
+  const size_t parts[] = {sizeof(Ctrl),      lay.lists_bytes, lay.blk_bytes, lay.bk8_bytes,
+                          lay.vflag_bytes,   lay.vlist_bytes, level[0],      level[1],
+                          level[2],          level[3]};
+  DevBuf *views[] = {&cb.gcount, &cb.lists,         &cb.blk,           &cb.bk8,
+                     &cb.vflag,  &cb.vlist,         &cb.scan.level[0], &cb.scan.level[1],
+                     &cb.scan.level[2], &cb.scan.level[3]};
+  size_t off = 0;
+  for (size_t b : parts) off += (b + 255) & ~(size_t)255;
+  if (need) *need = off;
+  if (off > bytes) return false;
+  off = 0;
+  for (size_t i = 0; i < sizeof(parts) / sizeof(parts[0]); i++) {
+    views[i]->borrow((uint8_t *)base + off, parts[i]);
+    off += (parts[i] + 255) & ~(size_t)255;
+  }
+  return true;
+}
+
+hipError_t clear_vflags(CountBuffers &cb, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_words, dim3(grid_for((uint64_t)n + 1, 256, 8192)), dim3(256), 0, st,
+                     cb.vflag.as<uint32_t>(), (uint64_t)n + 1, 0u);
+  return hipGetLastError();
+}
+
 hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
                          const DispatchParams *dp, CountBuffers &cb, bool classes,
                          int num_cus, hipStream_t st, const hipEvent_t *marks) {

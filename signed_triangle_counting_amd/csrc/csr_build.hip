@@ -494,6 +494,17 @@ __global__ void k_shift_ids(uint32_t *__restrict__ u, uint32_t *__restrict__ v, 
     if (e_ != hipSuccess) return e_;          \
   } while (0)
 
+size_t scan_level_bytes(uint64_t n, int l) {
+  uint64_t cur = n;
+  for (int k = 0; n && k < 4; k++) {
+    const uint64_t tiles = (cur + kScanTile - 1) / kScanTile;
+    if (k == l) return 2 * tiles * sizeof(uint32_t) + 256;
+    if (tiles <= 1) break;
+    cur = tiles;
+  }
+  return 0;
+}
+
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
                               ScanScratch &scr, hipStream_t s) {
   return [&]() -> hipError_t {
@@ -510,8 +521,7 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
     }
     if (sizes[levels - 1] > 1) return hipErrorInvalidValue;  // > 2048^4
     // level buffers: sums and scanned sums
-    for (int l = 0; l < levels; l++)
-      STC_TRY(scr.level[l].ensure(2 * sizes[l] * sizeof(uint32_t) + 256));
+    for (int l = 0; l < levels; l++) STC_TRY(scr.level[l].ensure(scan_level_bytes(n, l)));
     // down: scan tiles of input, collect sums
     const uint32_t *lin = in;
     uint32_t *lout = out;

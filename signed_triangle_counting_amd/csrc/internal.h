@@ -132,19 +132,31 @@ struct UploadRing {
 };
 
 // ---- device memory --------------------------------------------------------
+// A device buffer the context owns (grown by hipMalloc), or a borrowed view
+// of `bytes` bytes of memory someone else owns (the static scratch), which
+// never allocates: ensure() then fails when the view is too small.
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
+  }
+  void borrow(void *q, size_t n) {
+    release();
+    p = q;
+    bytes = n;
+    borrowed = true;
   }
   // Grow to at least `need` bytes (contents not preserved).
   hipError_t ensure(size_t need) {
     if (need == 0) need = 256;
     if (need <= bytes) return hipSuccess;
+    if (borrowed) return hipErrorInvalidValue;
     release();
     hipError_t e = hipMalloc(&p, need);
     if (e != hipSuccess) {
@@ -169,6 +181,8 @@ struct ScanScratch {
 // ---- launchers (return hipSuccess or the first error) -------------------
 hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
                               ScanScratch &scr, hipStream_t s);
+// Bytes of ScanScratch level l that a scan of n words needs (0: level unused).
+size_t scan_level_bytes(uint64_t n, int l);
 // Stable LSD radix sort of 64-bit keys on the given 8-bit digit shifts
 // (ascending significance).  Result lands in *keys or *tmp; on return *keys
 // points at the sorted data.
@@ -248,6 +262,17 @@ hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r,
                       hipStream_t st);
 size_t hash_lds_bytes(const TuneParams &t);
+// The counting pass's buffers for a graph of n vertices and m oriented edges,
+// carved from `bytes` bytes at `base` (256-byte aligned) as borrowed views;
+// false when they do not fit.  The vertex-flag words are cleared on st (they
+// must start at zero; the flag pass clears only what it set).
+constexpr size_t kStaticScratchBytes = (size_t)128 << 20;
+bool carve_count_buffers(void *base, size_t bytes, uint32_t n, uint64_t m, int num_cus,
+                         CountBuffers &cb, size_t *need);
+hipError_t clear_vflags(CountBuffers &cb, uint32_t n, hipStream_t st);
+// Device address of this library's static scratch array (kStaticScratchBytes)
+// on the current device.
+hipError_t static_scratch_address(void **ptr);
 size_t bitmap_lds_bytes(const TuneParams &t);
 size_t wave_bitmap_lds_bytes(const TuneParams &t);
 
