@@ -61,6 +61,14 @@ void stc_bucket_features(int bucket, uint32_t count, uint32_t n, uint64_t m, uin
 int stc_train_dispatcher(stc_ctx *ctx, const float *feat, const float *cost, int nsamp,
                          int iters, float lr, uint64_t seed, const char *out_path);
 
+/* The strategy per bucket (STC_TABLE_*) that the last count's dispatcher
+ * chose on the device -- the fixed rule, the MLP's argmax, a forced mode or a
+ * given table -- before the per-edge demotions of lists that do not fit the
+ * LDS structures; STC_TABLE_SKIP for empty buckets.  No counterpart in the
+ * reference (its 4x rule is decided per edge inside the kernel,
+ * signed_tc.hxx:47,65). */
+int stc_last_table(const stc_ctx *ctx, uint8_t out[STC_BUCKETS]);
+
 /* Reload "dispatch_mlp.txt" / "tune.txt" from dir (NULL: the shipped ones). */
 int stc_load_models(stc_ctx *ctx, const char *dir);
 

@@ -46,7 +46,7 @@ EXPORTED = [
     "stc_gen_er", "stc_gen_grid", "stc_gen_rmat", "stc_gen_hub", "stc_gen_batch",
     "stc_batch_free", "stc_gen_workload", "stc_gen_workload_batch",
     "stc_bucket_histogram", "stc_count_with_table", "stc_bucket_features",
-    "stc_train_dispatcher", "stc_load_models", "stc_set_tune",
+    "stc_train_dispatcher", "stc_load_models", "stc_set_tune", "stc_last_table",
 ]
 
 TABLE = {"skip": 0, "merge": 1, "hash": 2, "bitmap": 3, "pull": 4}
@@ -169,6 +169,7 @@ def lib():
                                        C.c_float, C.c_uint64, C.c_char_p]
     L.stc_load_models.argtypes = [C.c_void_p, C.c_char_p]
     L.stc_set_tune.argtypes = [C.c_void_p, C.c_char_p]
+    L.stc_last_table.argtypes = [C.c_void_p, C.c_void_p]
     _lib_handle = L
     return L
 
@@ -432,6 +433,12 @@ class Counter:
         assert f.shape[1] == 7 and c.shape[1] == 4 and f.shape[0] == c.shape[0]
         _check(lib().stc_train_dispatcher(self._h, f.ctypes.data, c.ctypes.data, f.shape[0],
                                           iters, lr, seed, out_path.encode()), self._h)
+
+    def last_table(self):
+        """The strategy per bucket (TABLE codes) the last count's dispatcher chose."""
+        out = np.zeros(256, np.uint8)
+        _check(lib().stc_last_table(self._h, out.ctypes.data), self._h)
+        return out
 
     def load_models(self, model_dir=None):
         rc = lib().stc_load_models(self._h, model_dir.encode() if model_dir else None)

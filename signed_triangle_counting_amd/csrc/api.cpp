@@ -54,6 +54,10 @@ struct stc_ctx {
   // that call failed.
   void *scratch = nullptr;
   hipError_t scratch_err = hipSuccess;
+  // strategy per bucket of the last count (STC_TABLE_* codes), before the
+  // per-edge demotions of lists that do not fit the LDS structures
+  uint8_t last_table[kBuckets] = {};
+  bool have_table = false;
 };
 
 namespace {
@@ -106,6 +110,10 @@ void default_tune(TuneParams &t) {
   t.bitmap_span_log2 = 17;
   t.wave_window_log2 = 15;
   t.hash_block = 256;
+  // untuned fixed rule: merge where both lists hold at most 8 entries, pull
+  // where the suffix is the shorter list, the reference's 4x rule, hub bitmap
+  t.fixed_merge_log2 = 3;
+  t.fixed_pull_log2 = 0;
   t.fixed_skew_log2 = 2;
   t.fixed_hub_log2 = 9;
 }
@@ -151,10 +159,20 @@ bool parse_tune(std::istream &f, TuneParams &t) {
     } else if (k == "wave_window_log2") {
       if (!(is >> x.wave_window_log2) || x.wave_window_log2 < 10 || x.wave_window_log2 > 16)
         return false;
+    } else if (k == "fixed_merge_log2") {
+      if (!(is >> x.fixed_merge_log2) || x.fixed_merge_log2 < -1 || x.fixed_merge_log2 > 15)
+        return false;
+    } else if (k == "fixed_pull_log2") {
+      if (!(is >> x.fixed_pull_log2) || x.fixed_pull_log2 < -1 || x.fixed_pull_log2 > 15)
+        return false;
     } else if (k == "fixed_skew_log2") {
-      if (!(is >> x.fixed_skew_log2)) return false;
+      if (!(is >> x.fixed_skew_log2) || x.fixed_skew_log2 < -1 || x.fixed_skew_log2 > 15)
+        return false;
     } else if (k == "fixed_hub_log2") {
-      if (!(is >> x.fixed_hub_log2)) return false;
+      if (!(is >> x.fixed_hub_log2) || x.fixed_hub_log2 < -1 || x.fixed_hub_log2 > 15)
+        return false;
+    } else {
+      return false;  // an unknown key: a table for another build
     }
   }
   t = x;
@@ -266,6 +284,19 @@ float median(std::vector<float> v) {
   std::sort(v.begin(), v.end());
   size_t k = v.size();
   return (k & 1) ? v[k / 2] : 0.5f * (v[k / 2 - 1] + v[k / 2]);
+}
+
+// The device table (Strategy values) as STC_TABLE_* codes.
+void keep_table(stc_ctx *c, const CountResult &res) {
+  for (int b = 0; b < kBuckets; b++) {
+    const uint8_t s = res.table[b];
+    c->last_table[b] = s == S_MERGE_LANE || s == S_MERGE_WAVE ? STC_TABLE_MERGE
+                       : s == S_HASH                          ? STC_TABLE_HASH
+                       : s == S_BITMAP                        ? STC_TABLE_BITMAP
+                       : s == S_PULL                          ? STC_TABLE_PULL
+                                                              : STC_TABLE_SKIP;
+  }
+  c->have_table = true;
 }
 
 int check_graph(stc_ctx *c, const stc_edges *g) {
@@ -458,6 +489,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   CountResult res;
   e = read_count(cb, classes, res, c->st);
   if (e != hipSuccess) return hip_fail(c, e, "count readback");
+  keep_table(c, res);
   if (classes) {
     c->classes = {res.t[0], res.t[1], res.t[2], res.t[3]};
     c->have_classes = true;
@@ -737,6 +769,7 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
   CountResult res;
   if (e == hipSuccess) e = read_count(c->cb, false, res, c->st);
   if (e != hipSuccess) return hip_fail(c, e, "batch count");
+  keep_table(c, res);
   total->balanced = total->unbalanced = 0;
   for (uint32_t i = 0; i < k; i++) {
     total->balanced += got[2 * i];
@@ -841,7 +874,10 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
+  CountResult res;
+  if (e == hipSuccess) e = read_count(c->cb, false, res, c->st);
   if (e != hipSuccess) return hip_fail(c, e, "batch graph");
+  keep_table(c, res);
   for (uint32_t i = 0; i < k; i++) {
     out[i].balanced = got[2 * i];
     out[i].unbalanced = got[2 * i + 1];
@@ -966,6 +1002,13 @@ extern "C" int stc_train_dispatcher(stc_ctx *c, const float *feat, const float *
   c->hp.model = m;
   c->hp.have_model = 1;
   c->model_status = STC_OK;
+  return STC_OK;
+}
+
+extern "C" int stc_last_table(const stc_ctx *c, uint8_t out[STC_BUCKETS]) {
+  if (!c || !out) return STC_E_ARG;
+  if (!c->have_table) return STC_E_ARG;
+  std::memcpy(out, c->last_table, kBuckets);
   return STC_OK;
 }
 

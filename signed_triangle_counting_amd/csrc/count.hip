@@ -97,11 +97,27 @@ __device__ __forceinline__ uint8_t merge_variant(int bk, const TuneParams &t) {
   return lmax <= t.merge_wave_min_log2[lmin] ? S_MERGE_LANE : S_MERGE_WAVE;
 }
 
+// The fixed rule: the fallback when no model is loaded, and the judge of the
+// learned dispatcher's speed.  Its thresholds are tuned (autotune.py) on the
+// dispatcher's training graphs; a negative threshold switches its branch off.
+//   1. Both lists short: merge (one lane, or one wave, per edge).
+//   2. The source's suffix shorter than the target's list by 2^fixed_pull:
+//      pull, which streams the suffix against the staged target list -- the
+//      reference's hash path iterates the shorter of the two lists against a
+//      table of the longer (tc_hash_sign_enu.cpp:138-144).
+//   3. Lengths apart by 2^fixed_skew (2^2: the reference GPU kernel's 4x rule,
+//      signed_tc.hxx:47,65, whose lanes binary-search the longer list): the
+//      wave merge, whose lanes binary-search a 64-entry tile of the other list.
+//   4. A long source suffix: push bitmap (the source list staged once, the
+//      target lists streamed).
+//   5. Otherwise merge.
 __device__ uint8_t fixed_rule(int bk, const TuneParams &t) {
-  int la = bk / kBucketSide, lb = bk % kBucketSide;
-  int skew = la > lb ? la - lb : lb - la;
-  if (skew >= t.fixed_skew_log2) return S_HASH;     // reference 4x skew rule
-  if (la >= t.fixed_hub_log2) return S_BITMAP;      // long source list
+  const int la = bk / kBucketSide, lb = bk % kBucketSide;
+  const int skew = la > lb ? la - lb : lb - la;
+  if (max(la, lb) <= t.fixed_merge_log2) return merge_variant(bk, t);
+  if (t.fixed_pull_log2 >= 0 && lb - la >= t.fixed_pull_log2) return S_PULL;
+  if (t.fixed_skew_log2 >= 0 && skew >= t.fixed_skew_log2) return S_MERGE_WAVE;
+  if (t.fixed_hub_log2 >= 0 && la >= t.fixed_hub_log2) return S_BITMAP;
   return merge_variant(bk, t);
 }
 

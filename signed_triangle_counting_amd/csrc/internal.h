@@ -65,8 +65,12 @@ struct TuneParams {
   int bitmap_span_log2;   // rank window of the workgroup bitmap (entries)
   int wave_window_log2;   // rank window of the per-wave bitmap (entries)
   int hash_block;         // threads per workgroup for hash/bitmap
-  int fixed_skew_log2;    // fixed rule: skew >= 2^x -> hash
-  int fixed_hub_log2;     // fixed rule: d+(v) >= 2^x (or source) -> bitmap
+  // The fixed rule (a negative threshold switches its branch off), on the
+  // bucket's la = ceil_log2 |N+(u) after v| and lb = ceil_log2 d+(v):
+  int fixed_merge_log2;   // max(la, lb) <= x: merge (lane or wave)
+  int fixed_pull_log2;    // lb - la >= x: pull (the shorter suffix streamed)
+  int fixed_skew_log2;    // |la - lb| >= x: wave merge (binary search of the longer)
+  int fixed_hub_log2;     // la >= x: push bitmap (the long source list staged)
   int allow_pull;         // 1: pull buckets are counted by the pull kernels (single
                           // graphs and batches); 0: they fall back to bitmap
 };
