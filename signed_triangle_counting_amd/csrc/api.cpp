@@ -116,6 +116,10 @@ void default_tune(TuneParams &t) {
   t.fixed_pull_log2 = 0;
   t.fixed_skew_log2 = 2;
   t.fixed_hub_log2 = 9;
+  t.pull_block_chunk_log2 = 11;
+  t.pull_wave_chunk_log2 = 8;
+  t.wave_pop = 8;
+  t.lane_item_log2 = 9;
 }
 
 std::string lib_dir() {
@@ -163,13 +167,26 @@ bool parse_tune(std::istream &f, TuneParams &t) {
       if (!(is >> x.fixed_merge_log2) || x.fixed_merge_log2 < -1 || x.fixed_merge_log2 > 15)
         return false;
     } else if (k == "fixed_pull_log2") {
-      if (!(is >> x.fixed_pull_log2) || x.fixed_pull_log2 < -1 || x.fixed_pull_log2 > 15)
+      if (!(is >> x.fixed_pull_log2) || x.fixed_pull_log2 < -15 || x.fixed_pull_log2 > 16)
         return false;
     } else if (k == "fixed_skew_log2") {
-      if (!(is >> x.fixed_skew_log2) || x.fixed_skew_log2 < -1 || x.fixed_skew_log2 > 15)
+      if (!(is >> x.fixed_skew_log2) || x.fixed_skew_log2 < 1 || x.fixed_skew_log2 > 16)
         return false;
     } else if (k == "fixed_hub_log2") {
-      if (!(is >> x.fixed_hub_log2) || x.fixed_hub_log2 < -1 || x.fixed_hub_log2 > 15)
+      if (!(is >> x.fixed_hub_log2) || x.fixed_hub_log2 < 0 || x.fixed_hub_log2 > 16)
+        return false;
+    } else if (k == "pull_block_chunk_log2") {
+      if (!(is >> x.pull_block_chunk_log2) || x.pull_block_chunk_log2 < 8 ||
+          x.pull_block_chunk_log2 > 14)
+        return false;
+    } else if (k == "pull_wave_chunk_log2") {
+      if (!(is >> x.pull_wave_chunk_log2) || x.pull_wave_chunk_log2 < 5 ||
+          x.pull_wave_chunk_log2 > 12)
+        return false;
+    } else if (k == "wave_pop") {
+      if (!(is >> x.wave_pop) || x.wave_pop < 1 || x.wave_pop > 8) return false;
+    } else if (k == "lane_item_log2") {
+      if (!(is >> x.lane_item_log2) || x.lane_item_log2 < 9 || x.lane_item_log2 > 12)
         return false;
     } else {
       return false;  // an unknown key: a table for another build

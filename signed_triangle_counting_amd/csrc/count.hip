@@ -99,7 +99,9 @@ __device__ __forceinline__ uint8_t merge_variant(int bk, const TuneParams &t) {
 
 // The fixed rule: the fallback when no model is loaded, and the judge of the
 // learned dispatcher's speed.  Its thresholds are tuned (autotune.py) on the
-// dispatcher's training graphs; a negative threshold switches its branch off.
+// dispatcher's training graphs; 16 switches the pull, skew and hub branches
+// off (la, lb <= 15), -1 the merge branch, and a negative pull threshold
+// also pulls where the suffix is the longer list.
 //   1. Both lists short: merge (one lane, or one wave, per edge).
 //   2. The source's suffix shorter than the target's list by 2^fixed_pull:
 //      pull, which streams the suffix against the staged target list -- the
@@ -115,9 +117,9 @@ __device__ uint8_t fixed_rule(int bk, const TuneParams &t) {
   const int la = bk / kBucketSide, lb = bk % kBucketSide;
   const int skew = la > lb ? la - lb : lb - la;
   if (max(la, lb) <= t.fixed_merge_log2) return merge_variant(bk, t);
-  if (t.fixed_pull_log2 >= 0 && lb - la >= t.fixed_pull_log2) return S_PULL;
-  if (t.fixed_skew_log2 >= 0 && skew >= t.fixed_skew_log2) return S_MERGE_WAVE;
-  if (t.fixed_hub_log2 >= 0 && la >= t.fixed_hub_log2) return S_BITMAP;
+  if (lb - la >= t.fixed_pull_log2) return S_PULL;
+  if (skew >= t.fixed_skew_log2) return S_MERGE_WAVE;
+  if (la >= t.fixed_hub_log2) return S_BITMAP;
   return merge_variant(bk, t);
 }
 
@@ -484,8 +486,9 @@ __global__ __launch_bounds__(256) void k_list_compact(const uint32_t *__restrict
 // wave pull (vertices with at most kWaveMaxDeg out-edges use the wave
 // kernels).  For pull the listed vertex is the target v whose list is
 // staged, and its entries are chunks of its in-edge range [tin_off[v],
-// tin_off[v + 1]): one entry per kPullBlockChunk (workgroup) or
-// kPullWaveChunk (wave) in-edges, holding the chunk's first position, so a
+// tin_off[v + 1]): one entry per 2^pull_block_chunk_log2 (workgroup) or
+// 2^pull_wave_chunk_log2 (wave) in-edges (tuned; 2048 and 256 by default),
+// holding the chunk's first position, so a
 // target with a huge in-degree is spread over many workgroups.  A workgroup
 // owns a tile of kVTile consecutive vertices, visited in rounds of 256:
 // k_vcount counts each list's entries per tile, an exclusive scan over the
@@ -493,8 +496,8 @@ __global__ __launch_bounds__(256) void k_list_compact(const uint32_t *__restrict
 // entries in order and clears the flags.
 constexpr uint32_t kVTile = 4096;
 constexpr int kVLists = 6;
-constexpr uint32_t kPullBlockChunk = 2048;
-constexpr uint32_t kPullWaveChunk = 256;
+// The smallest pull chunk the tune table admits (it bounds the list sizes).
+constexpr uint32_t kPullChunkMin = 32;
 
 // The strategy table by target class: pull_src[cb] has bit c set when bucket
 // (c, cb) is binned to pull (built in LDS by every workgroup of the pass;
@@ -540,8 +543,9 @@ __device__ __forceinline__ void vertex_lists_of(uint32_t v, const DevCsr &g, con
   c[1] = (f & 2u) && !small;
   c[2] = (f & 1u) && small;
   c[3] = (f & 2u) && small;
-  c[4] = pull && !small ? (indeg + kPullBlockChunk - 1) / kPullBlockChunk : 0u;
-  c[5] = pull && small ? (indeg + kPullWaveChunk - 1) / kPullWaveChunk : 0u;
+  const uint32_t cb_ = 1u << t.pull_block_chunk_log2, cw = 1u << t.pull_wave_chunk_log2;
+  c[4] = pull && !small ? (indeg + cb_ - 1) / cb_ : 0u;
+  c[5] = pull && small ? (indeg + cw - 1) / cw : 0u;
 }
 
 __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
@@ -555,9 +559,9 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
 }
 
 // Upper bound of the entries of one list: every vertex once plus the extra
-// chunks of the pull lists (at most m / kPullWaveChunk).
+// chunks of the pull lists (at most m / kPullChunkMin).
 __host__ __device__ inline uint64_t vlist_stride(uint32_t n, uint64_t m) {
-  return (uint64_t)n + 1 + m / kPullWaveChunk + 1;
+  return (uint64_t)n + 1 + m / kPullChunkMin + 1;
 }
 
 __global__ __launch_bounds__(256) void k_vcount(DevCsr g, const Ctrl *ctrl,
@@ -654,7 +658,8 @@ __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams
         if (L < 4) {
           dst[0] = v;
         } else {
-          const uint32_t chunk = L == 4 ? kPullBlockChunk : kPullWaveChunk;
+          const uint32_t chunk =
+              1u << (L == 4 ? tune.pull_block_chunk_log2 : tune.pull_wave_chunk_log2);
           const uint32_t t0 = g.tin_off[v];
           uint32_t *dst_t = dst + 2 * stride;  // the entries' targets
           for (uint32_t k = 0; k < c[L]; k++) {
@@ -758,6 +763,7 @@ __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
 // plus 4 for the graph_of read of each item in a batch.  List L's entries sit
 // at vlist + L vstride (targets of the pull lists 2 strides after them).
 __global__ __launch_bounds__(256) void k_list_stats(DevCsr g, Ctrl *ctrl,
+                                                    const DispatchParams *P,
                                                     const uint32_t *__restrict__ vlist,
                                                     uint64_t vstride, uint32_t batch) {
   __shared__ unsigned long long s_b[kVLists];
@@ -776,7 +782,8 @@ __global__ __launch_bounds__(256) void k_list_stats(DevCsr g, Ctrl *ctrl,
       } else {
         const uint32_t j0 = vlist[(uint64_t)L * vstride + i];
         const uint32_t v = vlist[(uint64_t)(L + 2) * vstride + i];
-        const uint32_t chunk = L == 4 ? kPullBlockChunk : kPullWaveChunk;
+        const uint32_t chunk =
+            1u << (L == 4 ? P->tune.pull_block_chunk_log2 : P->tune.pull_wave_chunk_log2);
         const uint32_t c = min(j0 + chunk, g.tin_off[v + 1]) - j0;
         const uint32_t d = g.off[v + 1] - g.off[v];
         acc += 28u + 4ull * d + 8ull * c + extra;
@@ -1723,7 +1730,7 @@ __device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, 
                                                   Acc<CLS> &acc) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nwaves = blockDim.x >> 6;
-  const PullChunk c = pull_chunk(g, j0, v, kPullBlockChunk);
+  const PullChunk c = pull_chunk(g, j0, v, 1u << tune.pull_block_chunk_log2);
   const Staged st = stage_block(g, c.v, B, L, s_nlow, 1u << tune.bitmap_span_log2);
   st.with_probe(B, L, [&](const auto &probe) {
     pull_stream<CLS, false>(g, c.t0 + wave, c.t1, nwaves, st.d, table, probe, nullptr, acc);
@@ -1783,7 +1790,7 @@ __global__ __launch_bounds__(kVertexBlock) void k_hash(DevCsr g, Ctrl *ctrl,
 }
 
 // One wave per source vertex (source lists of at most kWaveMaxDeg entries).
-constexpr uint32_t kWaveChunk = 8;  // vertices per queue pop
+constexpr uint32_t kWaveChunk = 8;  // the most items a wave takes per queue pop
 
 // Headers of the wave-pull chunks among the items of one queue pop, loaded
 // together so that an item does not start with a chain of dependent loads.
@@ -1840,7 +1847,8 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
   Acc<CLS> acc;
   while (true) {
     uint32_t first = 0, count = 0;
-    if (lane == 0 && !queue_pop<true>(ctrl->queue[2], len, kWaveChunk, cur, first, count))
+    if (lane == 0 &&
+        !queue_pop<true>(ctrl->queue[2], len, (uint32_t)tune.wave_pop, cur, first, count))
       count = 0;
     first = __shfl(first, 0, 64);
     count = __shfl(count, 0, 64);
@@ -1911,7 +1919,8 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   while (true) {
     uint32_t first = 0, count = 0;
     if (lane == 0 &&
-        !queue_pop<true>(ctrl->queue[PULL ? 6 : 4], len, kWaveChunk, cur, first, count))
+        !queue_pop<true>(ctrl->queue[PULL ? 6 : 4], len, (uint32_t)tune.wave_pop, cur, first,
+                         count))
       count = 0;
     first = __shfl(first, 0, 64);
     count = __shfl(count, 0, 64);
@@ -1921,7 +1930,7 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
     for (uint32_t it = 0; it < count; it++) {
       if constexpr (PULL) {
         const uint32_t j0 = head_of(heads, 0, it);
-        pull_vertex_wave<CLS>(g, j0, min(j0 + kPullWaveChunk, head_of(heads, 4, it)),
+        pull_vertex_wave<CLS>(g, j0, min(j0 + (1u << tune.pull_wave_chunk_log2), head_of(heads, 4, it)),
                               head_of(heads, 2, it), head_of(heads, 3, it), B, L, window,
                               s_heads + wave * 64u, table, acc);
       } else {
@@ -1950,9 +1959,10 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 // reads the list lengths and the one whose kinds are absent returns at once.
 // Two phases per workgroup, each draining one queue:
 //   1. workgroup items, popped by the whole workgroup (heaviest kinds first):
-//      merge-lane chunks of kLaneItem edges (one per thread), hash vertices,
+//      merge-lane chunks of 2^lane_item_log2 edges (tuned; 512 by default,
+//      one per thread), hash vertices,
 //      bitmap vertices, workgroup-pull in-edge chunks;
-//   2. wave items, popped by each wave on its own (kWaveChunk per pop):
+//   2. wave items, popped by each wave on its own (wave_pop per pop):
 //      merge-wave edges, wave-hash and wave-bitmap vertices, wave-pull
 //      in-edge chunks.  A wave never waits for the others, so short and long
 //      items mix without idling the workgroup.
@@ -1973,7 +1983,6 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 // no clearing: phase 1, which needs zeros there, is over by then).  Bitmaps
 // clear what they set; tables and low arrays are cleared after use.
 constexpr uint32_t kBatchBlock = 512;
-constexpr uint32_t kLaneItem = kBatchBlock;
 constexpr uint32_t kWavesPerBatchBlock = kBatchBlock / 64;
 constexpr int kKindAll = 0, kKindPullBatch = 1, kKindPullOne = 2;
 
@@ -2033,26 +2042,31 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
   if (tid < kBuckets) table[tid] = ctrl->table[tid];
   if (tid == 0) tune = P->tune;
   for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
+  __syncthreads();  // table, tune and the zeroed region are visible to every wave
   GraphAcc<CLS> ga;
   Acc<CLS> &acc = ga.acc;
   // phase 1: workgroup items [lane chunks | hash | bitmap | pull chunks]
   {
-    const uint32_t b1 = (nl + kLaneItem - 1) / kLaneItem, b2 = b1 + nh, b3 = b2 + nb,
+    const uint32_t lane_item = 1u << tune.lane_item_log2;
+    const uint32_t b1 = (nl + lane_item - 1) / lane_item, b2 = b1 + nh, b3 = b2 + nb,
                    total = b3 + np;
     QueueCursor cur{blockIdx.x & 7u, 0u};
     uint32_t it;
     while (block_pop(ctrl->queue[q1], total, cur, &s_first, &s_count, it)) {
-      if (ALL && it < b1) {  // merge-lane chunk
-        const uint32_t i = it * kLaneItem + tid;
-        const bool valid = i < nl;
-        uint32_t c[4] = {0, 0, 0, 0};
-        uint32_t gid = 0;
-        if (valid) {
-          const uint32_t e = lane_list[i];
-          gid = graph_of[g.src[e]];
-          merge_lane_edge<false, false>(g, e, c);  // batches: two-class
+      if (ALL && it < b1) {  // merge-lane chunk: kBatchBlock edges per round
+        const uint32_t end = min((it + 1) * lane_item, nl);
+        for (uint32_t i0 = it * lane_item; i0 < end; i0 += kBatchBlock) {
+          const uint32_t i = i0 + tid;
+          const bool valid = i < end;
+          uint32_t c[4] = {0, 0, 0, 0};
+          uint32_t gid = 0;
+          if (valid) {
+            const uint32_t e = lane_list[i];
+            gid = graph_of[g.src[e]];
+            merge_lane_edge<false, false>(g, e, c);  // batches: two-class
+          }
+          lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
         }
-        lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
       } else if (ALL && it < b2) {  // hash vertex
         const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[it - b1]);
         ga.to(per_graph, graph_of[a]);
@@ -2086,8 +2100,9 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
       // a batch keeps each XCD on a contiguous range (its graphs stay in one
       // L2); one graph's pops are dealt round-robin over the XCDs, which
       // spreads the heavy top of the list (as in the single-graph wave kernels)
-      if (lane == 0 && !queue_pop<KIND == kKindPullOne>(ctrl->queue[q2], total, kWaveChunk, cur,
-                                                         first, count))
+      if (lane == 0 && !queue_pop<KIND == kKindPullOne>(ctrl->queue[q2], total,
+                                                         (uint32_t)tune.wave_pop, cur, first,
+                                                         count))
         count = 0;
       first = __shfl(first, 0, 64);
       count = __shfl(count, 0, 64);
@@ -2118,7 +2133,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
           const uint32_t i = it - first;
           const uint32_t j0 = head_of(heads, 0, i);
           ga.to(per_graph, head_of(heads, 5, i));
-          pull_vertex_wave<CLS>(g, j0, min(j0 + kPullWaveChunk, head_of(heads, 4, i)),
+          pull_vertex_wave<CLS>(g, j0, min(j0 + (1u << tune.pull_wave_chunk_log2), head_of(heads, 4, i)),
                                   head_of(heads, 2, i), head_of(heads, 3, i), W, LW, window,
                                   HW, table, acc);
           LW[lane] = 0u;
@@ -2436,7 +2451,7 @@ hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers
   bin_lists(bin_layout(g.n, g.m, num_cus), g.m, cb.lists.as<uint32_t>(), cb.vlist.as<uint32_t>(),
             L);
   hipLaunchKernelGGL(k_list_stats, dim3(num_cus * 4), dim3(256), 0, st, g, cb.gcount.as<Ctrl>(),
-                     L.vh, L.vstride, batch ? 1u : 0u);
+                     dp, L.vh, L.vstride, batch ? 1u : 0u);
   return hipGetLastError();
 }
 

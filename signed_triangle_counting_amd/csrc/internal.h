@@ -65,12 +65,17 @@ struct TuneParams {
   int bitmap_span_log2;   // rank window of the workgroup bitmap (entries)
   int wave_window_log2;   // rank window of the per-wave bitmap (entries)
   int hash_block;         // threads per workgroup for hash/bitmap
-  // The fixed rule (a negative threshold switches its branch off), on the
-  // bucket's la = ceil_log2 |N+(u) after v| and lb = ceil_log2 d+(v):
-  int fixed_merge_log2;   // max(la, lb) <= x: merge (lane or wave)
-  int fixed_pull_log2;    // lb - la >= x: pull (the shorter suffix streamed)
+  // The fixed rule, on the bucket's la = ceil_log2 |N+(u) after v| and
+  // lb = ceil_log2 d+(v), both <= 15 (so 16 switches a branch off):
+  int fixed_merge_log2;   // max(la, lb) <= x: merge (lane or wave); -1: off
+  int fixed_pull_log2;    // lb - la >= x: pull (the suffix streamed); -15..16
   int fixed_skew_log2;    // |la - lb| >= x: wave merge (binary search of the longer)
   int fixed_hub_log2;     // la >= x: push bitmap (the long source list staged)
+  // Item sizes of the vertex-grouped and batched kernels:
+  int pull_block_chunk_log2;  // in-edges per workgroup-pull item (8..14)
+  int pull_wave_chunk_log2;   // in-edges per wave-pull item (5..12)
+  int wave_pop;               // items a wave takes per queue pop (1..8)
+  int lane_item_log2;         // merge-lane edges per batched workgroup item (9..12)
   int allow_pull;         // 1: pull buckets are counted by the pull kernels (single
                           // graphs and batches); 0: they fall back to bitmap
 };

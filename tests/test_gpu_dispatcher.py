@@ -183,17 +183,18 @@ def test_learned_table_is_torch_argmax(ctx, case):
 
 
 def _fixed_table(tune, hist):
-    """The fixed rule of the shipped tune.txt, bucket by bucket."""
+    """The fixed rule of the shipped tune.txt, bucket by bucket (16 switches
+    the pull, skew and hub branches off)."""
     t = np.zeros(256, np.uint8)
     for b in np.nonzero(hist)[0]:
         la, lb = int(b) // 16, int(b) % 16
         if max(la, lb) <= tune["fixed_merge_log2"]:
             s = "merge"
-        elif tune["fixed_pull_log2"] >= 0 and lb - la >= tune["fixed_pull_log2"]:
+        elif lb - la >= tune["fixed_pull_log2"]:
             s = "pull"
-        elif tune["fixed_skew_log2"] >= 0 and abs(la - lb) >= tune["fixed_skew_log2"]:
+        elif abs(la - lb) >= tune["fixed_skew_log2"]:
             s = "merge"  # the wave merge
-        elif tune["fixed_hub_log2"] >= 0 and la >= tune["fixed_hub_log2"]:
+        elif la >= tune["fixed_hub_log2"]:
             s = "bitmap"
         else:
             s = "merge"
