@@ -69,7 +69,8 @@ int stc_train_dispatcher(stc_ctx *ctx, const float *feat, const float *cost, int
  * signed_tc.hxx:47,65). */
 int stc_last_table(const stc_ctx *ctx, uint8_t out[STC_BUCKETS]);
 
-/* Reload "dispatch_mlp.txt" / "tune.txt" from dir (NULL: the shipped ones). */
+/* Reload "dispatch_mlp.txt" / "tune.txt" from dir (NULL: the shipped ones).
+ * A directory without a valid tune.txt keeps the context's tune values. */
 int stc_load_models(stc_ctx *ctx, const char *dir);
 
 /* Apply tune parameters given in the tune.txt text format. */

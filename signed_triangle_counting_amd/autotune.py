@@ -49,7 +49,7 @@ KEYS = ["merge_wave_min_log2", "hash_max_log2", "bitmap_span_log2", "wave_window
         "pull_block_chunk_log2", "pull_wave_chunk_log2", "wave_pop", "lane_item_log2"]
 
 DEFAULT = {"merge_wave_min_log2": [5] * 16, "hash_max_log2": 10, "bitmap_span_log2": 17,
-           "wave_window_log2": 15, "fixed_merge_log2": 3, "fixed_pull_log2": 0,
+           "wave_window_log2": 14, "fixed_merge_log2": 3, "fixed_pull_log2": 0,
            "fixed_skew_log2": 2, "fixed_hub_log2": 9, "pull_block_chunk_log2": 11,
            "pull_wave_chunk_log2": 8, "wave_pop": 8, "lane_item_log2": 9}
 

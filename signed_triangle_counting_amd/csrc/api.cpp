@@ -108,7 +108,7 @@ void default_tune(TuneParams &t) {
   for (int i = 0; i < kBucketSide; i++) t.merge_wave_min_log2[i] = 5;
   t.hash_max_log2 = 10;
   t.bitmap_span_log2 = 17;
-  t.wave_window_log2 = 15;
+  t.wave_window_log2 = 14;
   t.hash_block = 256;
   // untuned fixed rule: merge where both lists hold at most 8 entries, pull
   // where the suffix is the shorter list, the reference's 4x rule, hub bitmap
@@ -1032,10 +1032,10 @@ extern "C" int stc_last_table(const stc_ctx *c, uint8_t out[STC_BUCKETS]) {
 extern "C" int stc_load_models(stc_ctx *c, const char *dir) {
   if (!c) return STC_E_ARG;
   std::string d = dir ? std::string(dir) : lib_dir() + "/models";
+  // a directory without a (valid) tune.txt keeps the context's tune values
   TuneParams t;
   default_tune(t);
-  load_tune(d + "/tune.txt", t);
-  c->hp.tune = t;
+  if (load_tune(d + "/tune.txt", t)) c->hp.tune = t;
   std::string why;
   if (load_model(d + "/dispatch_mlp.txt", c->hp.model, why)) {
     c->hp.have_model = 1;

@@ -169,7 +169,9 @@ def main():
     tuned = {}
     F = None
     if args.eval_model:
-        ctx.load_models(args.eval_model)
+        if ctx.load_models(args.eval_model) != 0:
+            raise SystemExit("no valid dispatch_mlp.txt in %s: %s" % (args.eval_model,
+                                                                    ctx.last_error()))
     else:
         if args.samples:
             with np.load(args.samples) as z:
@@ -191,7 +193,8 @@ def main():
         path = os.path.join(args.out, "dispatch_mlp.txt")
         ctx.train_dispatcher(F, Cst, path, iters=args.iters, lr=args.lr, seed=args.seed)
         print("trained ->", path, flush=True)
-        ctx.load_models(args.out)
+        if ctx.load_models(args.out) != 0:
+            raise SystemExit("the trained model does not load: " + ctx.last_error())
     # every dispatch mode on every training graph and batch (and equal counts)
     rows = evaluate(ctx, graphs, log=lambda s: print(s, flush=True))
     rows += evaluate(ctx, batches, log=lambda s: print(s, flush=True), batch=True)
