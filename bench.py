@@ -1,24 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the signed triangle counting hot path on one MI355X.
 
-Workload (default): RMAT scale 22, edge factor 16, (a,b,c,d) =
+Headline workload: RMAT scale 22, edge factor 16, (a,b,c,d) =
 (0.57,0.19,0.19,0.05), signs -1 with probability 0.25 -- 67,108,864 generated
 lines, 64,155,555 oriented edges after dedupe.  The graph is generated on the
-host, uploaded and turned into the oriented CSR on the device once (reported
-as build_ms); a step is one pass of the counting phase (binning + every
-intersection kernel) over that device-resident CSR.
+host, uploaded and turned into the oriented CSR (and the transposed index the
+pull kernels read) on the device once, reported as build_ms; a step is one
+pass of the counting phase (binning + every intersection kernel) over that
+device-resident CSR -- the reference's "Triangle Counting Execution" boundary
+(Multicore Implementations/tc_merge_sign_enu.cpp:131-145), which also leaves
+the CSR build out.
 
 The JSON line carries:
-  value       oriented edges / s of the counting phase (wall clock over the
-              timed steps, bracketed by device synchronisation)
-  roofline    the dominant kernel's algorithmic bytes (4(d+(u)+d+(v)) per
-              oriented edge it counted) / its mean hipEvent duration, against
-              the 8 TB/s HBM peak; traffic = FETCH_SIZE + WRITE_SIZE bytes per
-              launch from the committed rocprofv3 PMC summary, if present
-  cpu_baseline the CPU oracle's merge count (a restatement of the reference's
-              OpenMP merge path) timed on this host over a bounded sample
-  learned_vs_fixed  counting time of the fixed-threshold dispatch over the
-              learned dispatch on the hub-dominated graph (18M vertices)
+  value         oriented edges / s of the counting phase (wall clock over the
+                timed steps, bracketed by device synchronisation)
+  roofline      the dominant kernel's algorithmic bytes -- what its strategy
+                must read (alg_bytes_kernel) -- over its median hipEvent
+                duration, against the 8 TB/s HBM peak; traffic = FETCH_SIZE +
+                WRITE_SIZE bytes per launch from the committed rocprofv3 PMC
+                summary (profiles/traffic.json); the merge-model B_alg of the
+                whole phase beside it, for information
+  cpu_baseline  the oracle's merge count (the reference's OpenMP merge path
+                restated) on this host, OMP threads = nproc, bounded sample
+  configs       one record per configuration (ER 10k/100k, grid 960^2, RMAT-22,
+                hub 18M, the 4,096-graph batch): event-timed count (median of
+                the timed repeats after warm-up), oriented edges/s, dominant
+                kernel roofline, counter traffic, every dispatch mode's time,
+                learned / tuned-fixed and learned / best-forced, the reference
+                goldens, and the CPU merge and hash paths (oracle) on this host
+  learned_vs_fixed  the hub-dominated graph's record, repeated at top level
 """
 import argparse
 import json
@@ -32,7 +42,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 KERNELS = ["merge_lane", "merge_wave", "hash_wave", "hash", "bitmap_wave", "bitmap",
            "pull_wave", "pull"]
-WORKLOADS = {1: "er_10k_100k", 2: "grid_960x960_diag", 3: "rmat22_ef16", 4: "hub_18m"}
+WORKLOADS = {1: "er_10k_100k", 2: "grid_960x960_diag", 3: "rmat22_ef16", 4: "hub_18m",
+             5: "batch4096"}
+MODES = ("learned", "fixed", "merge", "hash", "bitmap", "pull")
+FORCED = ("merge", "hash", "bitmap", "pull")
+TIMED_REGION = ("binning + intersection kernels over the device-resident oriented CSR and its "
+                "transposed index, both built once by stc_prepare (build_ms); the reference's "
+                "counting phase (tc_merge_sign_enu.cpp:131-145) likewise excludes its CSR build")
 
 
 def parse_args():
@@ -40,18 +56,21 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", type=int, default=3)
     ap.add_argument("--dispatch", default="learned")
+    ap.add_argument("--reps", type=int, default=7,
+                    help="event-timed repeats per dispatch mode in the config records")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-hub", action="store_true")
-    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="only the headline RMAT-22 line (no per-config records)")
+    ap.add_argument("--profile-config", type=int, default=0,
+                    help="count only config N (1-5) with learned dispatch, for rocprofv3")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="measure the HBM read rate for sequential and random-segment reads")
-    ap.add_argument("--batch-only", action="store_true",
-                    help="count only the 4,096-graph batch (for profiling) and print its record")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--traffic-from", default=None,
                     help="write profiles/traffic.json from a rocprofv3 PMC directory and exit")
+    ap.add_argument("--workload", type=int, default=3,
+                    help="with --traffic-from: the config the PMC runs counted")
     return ap.parse_args()
 
 
@@ -88,14 +107,27 @@ def cpu_model():
     return None
 
 
+def cpu_threads():
+    """OMP threads as the reference runs them: OMP_NUM_THREADS, else $(nproc)
+    (README.md:8)."""
+    nproc = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return int(env) if env and env.isdigit() and int(env) > 0 else nproc, nproc
+
+
+def median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2] if xs else None
+
+
 def cpu_baseline(g, budget_s):
     """The oracle's merge count (tc_merge_sign_enu.cpp:131-145 restated), the
     median of 3 timed runs over every stride-th source vertex; a first pass
     over 1/64 of the vertices sets the stride so each run takes about a third
     of budget_s seconds (the whole graph when that fits).  Returns oriented
-    edges per second."""
+    edges per second (and the oracle's CSR, for the hash path's timing)."""
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    threads, nproc = cpu_threads()
     t0 = time.time()
     csr = oracle.Csr(g.n, g.u, g.v, g.sign, threads)
     build_s = time.time() - t0
@@ -105,11 +137,11 @@ def cpu_baseline(g, budget_s):
     runs = [csr.merge_sample(stride, (1 + k) % stride, threads) for k in range(3)]
     rates = sorted(r["edges"] / r["seconds"] for r in runs if r["seconds"] > 0)
     mid = runs[1]
-    return {
+    return csr, {
         "value": rates[len(rates) // 2] if rates else None,
         "unit": "oriented edges/s",
         "cores": threads,
-        "nproc": os.cpu_count(),
+        "nproc": nproc,
         "cpu_model": cpu_model(),
         "kind": "port",
         "runs": [round(r, 1) for r in rates],
@@ -118,6 +150,45 @@ def cpu_baseline(g, budget_s):
                   "CSR build (%.1f s) not included"
                   % (threads, stride, mid["edges"], csr.m, mid["seconds"], build_s),
     }
+
+
+def cpu_paths(graphs):
+    """The reference's two OpenMP counting paths restated by the oracle, on
+    this host with OMP threads = nproc: the merge path's counting loop
+    (tc_merge_sign_enu.cpp:131-145) and the hash path's (tc_hash_sign_enu.cpp:
+    124-165, its table build timed apart), median of 3 runs each, summed over
+    the given graphs; the oracle's CSR build reported apart."""
+    import oracle
+    threads, nproc = cpu_threads()
+    build = merge_s = hash_s = hash_build = 0.0
+    m = 0
+    counts = [0, 0]
+    for g in graphs:
+        t0 = time.time()
+        csr = oracle.Csr(g.n, g.u, g.v, g.sign, threads)
+        build += time.time() - t0
+        m += csr.m
+        mt, hb, hc = [], [], []
+        for _ in range(3):
+            t, b, u = csr.time_merge(threads)
+            mt.append(t)
+            th, bs, b2, u2 = csr.time_hash(threads)
+            hc.append(th)
+            hb.append(bs)
+            if (b, u) != (b2, u2):
+                raise RuntimeError("oracle merge and hash paths disagree")
+        counts[0] += b
+        counts[1] += u
+        merge_s += median(mt)
+        hash_s += median(hc)
+        hash_build += median(hb)
+        del csr
+    return {"threads": threads, "nproc": nproc, "m_oriented": m,
+            "merge_s": round(merge_s, 4),
+            "merge_edges_per_s": m / merge_s if merge_s > 0 else None,
+            "hash_s": round(hash_s, 4), "hash_table_build_s": round(hash_build, 4),
+            "hash_edges_per_s": m / hash_s if hash_s > 0 else None,
+            "csr_build_s": round(build, 3), "counts": counts}
 
 
 def traffic_from(pmc_dir, workload, out_path):
@@ -150,8 +221,8 @@ def traffic_from(pmc_dir, workload, out_path):
              "k_hash_wave": "hash_wave", "k_hash<": "hash",
              "k_bitmap_wave<false, false>": "bitmap_wave", "k_bitmap<false>": "bitmap",
              "k_bitmap_wave<false, true>": "pull_wave",
-             # a two-class count's workgroup and wave pull chunks share one
-             # two-phase launch (the batch kernel, as a batch of one graph)
+             # one graph's workgroup and wave pull chunks, or a whole batch,
+             # share one two-phase launch (reported in the last slot)
              "k_two_phase": "pull"}
     out = {}
     try:
@@ -169,12 +240,32 @@ def traffic_from(pmc_dir, workload, out_path):
         for wn, wd in write.items():
             if wn == name and "WRITE_SIZE" in wd:
                 wk = sum(wd["WRITE_SIZE"]) / len(wd["WRITE_SIZE"])
-        rec[key] = {"bytes_per_launch": round(1024.0 * (fk * ff + wk * fw)),
-                    "fetch_kib_raw": fk, "write_kib_raw": wk}
+        prev = rec.get(key)
+        # the launch that did the work (the others of the slot exit at once)
+        if prev is None or fk > prev["fetch_kib_raw"]:
+            rec[key] = {"kernel": name.split("(")[0], "bytes_per_launch":
+                        round(1024.0 * (fk * ff + wk * fw)),
+                        "fetch_bytes": round(1024.0 * fk * ff), "write_bytes": round(1024.0 * wk * fw),
+                        "fetch_kib_raw": fk, "write_kib_raw": wk}
     out[workload] = dict(rec, _calibration={"fetch_factor": ff, "write_factor": fw})
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out[workload], indent=1))
+
+
+def hbm_probe(device):
+    """HBM read rate (TB/s) for a sequential read and for random 128-byte-aligned
+    segments of 128..1024 bytes, from the bench-only probe library (not part of
+    libstc.so)."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "signed_triangle_counting_amd", "libstc_bench_probe.so"))
+    lib.bench_probe_read_bandwidth.argtypes = [C.c_int, C.POINTER(C.c_double)]
+    out = (C.c_double * 5)()
+    rc = lib.bench_probe_read_bandwidth(device, out)
+    if rc != 0:
+        raise RuntimeError("bench_probe_read_bandwidth returned %d" % rc)
+    keys = ("sequential", "segment_128B", "segment_256B", "segment_512B", "segment_1024B")
+    return {k: round(float(v), 3) for k, v in zip(keys, out)}
 
 
 def roofline_of(st, wname):
@@ -196,6 +287,8 @@ def roofline_of(st, wname):
         "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
         "traffic": traffic,
         "traffic_over_algorithmic": round(traffic / alg, 3) if traffic and alg else None,
+        "traffic_rate_frac": (round(traffic / (kms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+                              if traffic and kms > 0 else None),
         "kernel_ms": round(kms, 4), "algorithmic_bytes": alg,
         "count_phase_merge_model": {
             "bytes": st["b_alg"], "achieved": round(phase, 1),
@@ -205,50 +298,89 @@ def roofline_of(st, wname):
     }
 
 
-def hbm_probe(device):
-    """HBM read rate (TB/s) for a sequential read and for random 128-byte-aligned
-    segments of 128..1024 bytes, from the bench-only probe library (not part of
-    libstc.so)."""
-    import ctypes as C
-    lib = C.CDLL(os.path.join(ROOT, "signed_triangle_counting_amd", "libstc_bench_probe.so"))
-    lib.bench_probe_read_bandwidth.argtypes = [C.c_int, C.POINTER(C.c_double)]
-    out = (C.c_double * 5)()
-    rc = lib.bench_probe_read_bandwidth(device, out)
-    if rc != 0:
-        raise RuntimeError("bench_probe_read_bandwidth returned %d" % rc)
-    keys = ("sequential", "segment_128B", "segment_256B", "segment_512B", "segment_1024B")
-    return {k: round(float(v), 3) for k, v in zip(keys, out)}
-
-
-def hub_learned_vs_fixed(ctx, stc, steps):
-    g = stc.gen_workload(4)
-    ctx.prepare(g)
-    out = {"workload": WORKLOADS[4], "m_oriented": ctx.stats()["m_oriented"]}
-    res = {}
-    for mode in ("fixed", "learned"):
-        ctx.count_prepared(dispatch=mode, repeat=2)
-        res[mode] = ctx.count_prepared(dispatch=mode, repeat=steps)
-        out[mode + "_ms"] = round(ctx.stats()["ms_count"], 4)
-    out["exact_equal"] = res["fixed"] == res["learned"]
-    out["speedup"] = round(out["fixed_ms"] / out["learned_ms"], 3) if out["learned_ms"] else None
+def time_modes(ctx, reps, warmup, batch=False):
+    """Event-timed counting phase under every dispatch mode: {mode: (ms, counts)}
+    (single graph: median of `reps` runs after `warmup`; batch: median of
+    `reps` hipGraph replays after the eager runs)."""
+    out = {}
+    for mode in MODES:
+        if batch:
+            cnt = tuple(map(tuple, ctx.count_batch_prepared(dispatch=mode, repeat=reps)))
+        else:
+            if warmup:
+                ctx.count_prepared(dispatch=mode, repeat=warmup)
+            cnt = ctx.count_prepared(dispatch=mode, repeat=reps)
+        out[mode] = (ctx.stats()["ms_count"], cnt)
     return out
 
 
-def batch_workload(ctx, stc, steps):
-    """4,096 independent graphs counted in one persistent launch per step,
-    replayed from a captured hipGraph; per-graph counts against the fixture."""
-    b = stc.gen_workload_batch(4096)
-    t0 = time.perf_counter()
-    got = ctx.count_batch(b, dispatch="learned", repeat=steps)
-    wall = time.perf_counter() - t0
+def config_record(ctx, wid, g, reps, warmup, batch=False):
+    """One configuration: prepare, time every dispatch mode, then the learned
+    dispatch's statistics and roofline."""
+    wname = WORKLOADS[wid]
+    if batch:
+        ctx.prepare_batch(g)
+    else:
+        ctx.prepare(g)
+    st0 = ctx.stats()
+    res = time_modes(ctx, reps, warmup, batch)
+    counts = {v[1] for v in res.values()}
+    if len(counts) != 1:
+        raise RuntimeError("dispatch modes disagree on %s" % wname)
+    # learned last, so the statistics below are the learned dispatch's
+    if batch:
+        got = ctx.count_batch_prepared(dispatch="learned", repeat=reps)
+    else:
+        got = ctx.count_prepared(dispatch="learned", repeat=reps)
     st = ctx.stats()
-    per = goldens().get("batch", {}).get("per_graph")
-    exact = None if per is None else all(list(got[i]) == per[i][1:3] for i in range(b.k))
-    return {"graphs": b.k, "lines": b.total_m, "m_oriented": st["m_oriented"],
-            "ms_per_replay": round(st["ms_count"], 4),
-            "oriented_edges_per_s": st["m_oriented"] / (st["ms_count"] * 1e-3),
-            "build_ms": round(st["ms_build"], 3), "upload_ms": round(st["ms_upload"], 3),
-            "call_wall_s": round(wall, 3), "exact_vs_reference": exact}
+    ms = {m: round(v[0], 4) for m, v in res.items()}
+    best = min(FORCED, key=lambda m: res[m][0])
+    gold = goldens()
+    if batch:
+        per = gold.get("batch", {}).get("per_graph")
+        exact = None if per is None else all(list(got[i]) == per[i][1:3] for i in range(len(got)))
+        bal, unb = sum(x[0] for x in got), sum(x[1] for x in got)
+    else:
+        gw = gold.get("workloads", {}).get(str(wid))
+        exact = None if gw is None else (got == (gw["balanced"], gw["unbalanced"]))
+        bal, unb = got
+    t = res["learned"][0]
+    return {
+        "workload": wname, "n": st0["n"], "lines": st0["m_in"], "m_oriented": st0["m_oriented"],
+        "graphs": len(got) if batch else 1,
+        "counts": {"balanced": bal, "unbalanced": unb, "exact_vs_reference": exact},
+        "t_count_ms": round(t, 4), "oriented_edges_per_s": st0["m_oriented"] / (t * 1e-3),
+        "ms_by_dispatch": ms,
+        "learned_vs_fixed": round(res["fixed"][0] / t, 3),
+        "learned_vs_best_forced": round(res[best][0] / t, 3), "best_forced": best,
+        "ms_bin": round(st["ms_bin"], 4),
+        "roofline": roofline_of(st, wname),
+        "edges_by_strategy": {"merge": st["edges_merge"], "hash": st["edges_hash"],
+                              "bitmap": st["edges_bitmap"], "pull": st["edges_pull"],
+                              "skipped": st["edges_skipped"]},
+        "build_ms": round(st0["ms_build"], 3), "upload_ms": round(st0["ms_upload"], 3),
+        "timing": ("hipEvent median of %d %s" % (reps, "hipGraph replays after 2 eager runs"
+                                                 if batch else "runs after %d warm-up" % warmup)),
+    }
+
+
+def profile_config(wid, steps, warmup):
+    """Count only config wid under learned dispatch (for rocprofv3 runs)."""
+    import signed_triangle_counting_amd as stc
+    with stc.Counter(0) as ctx:
+        if wid == 5:
+            b = stc.gen_workload_batch(4096)
+            ctx.prepare_batch(b)
+            got = ctx.count_batch_prepared(dispatch="learned", repeat=steps)
+            print(json.dumps({"workload": WORKLOADS[5], "graphs": len(got),
+                              "ms_per_replay": ctx.stats()["ms_count"]}), flush=True)
+        else:
+            g = stc.gen_workload(wid)
+            ctx.prepare(g)
+            ctx.count_prepared(repeat=max(warmup, 1))
+            got = ctx.count_prepared(repeat=steps)
+            print(json.dumps({"workload": WORKLOADS[wid], "counts": got,
+                              "ms_count": ctx.stats()["ms_count"]}), flush=True)
 
 
 def main():
@@ -268,13 +400,14 @@ def main():
         print(json.dumps({"hbm_read_tbps": hbm_probe(0),
                           "peak_tbps": HBM_PEAK_GBPS / 1000.0}), flush=True)
         return
-    if args.batch_only:
-        with stc.Counter(0) as ctx:
-            print(json.dumps({"batch": batch_workload(ctx, stc, args.steps)}), flush=True)
+    if args.profile_config:
+        profile_config(args.profile_config, args.steps, args.warmup)
         return
-    wname = WORKLOADS[args.workload]
-    g = stc.gen_workload(args.workload)
+    wname = WORKLOADS[3]
+    g = stc.gen_workload(3)
     ctx = stc.Counter(0)
+    if ctx.model_status != 0:
+        raise RuntimeError("dispatcher model not loaded: " + ctx.last_error())
     ctx.prepare(g)
     st0 = ctx.stats()
     if args.warmup > 0:
@@ -287,10 +420,8 @@ def main():
     st = ctx.stats()
     ms_per_step = 1e3 * wall / args.steps
     m_plus = st["m_oriented"]
-    gold = goldens().get("workloads", {}).get(str(args.workload))
+    gold = goldens().get("workloads", {}).get("3")
     exact = None if gold is None else (counts == (gold["balanced"], gold["unbalanced"]))
-
-    roofline = roofline_of(st, wname)
     line = {
         "metric": "oriented_edges_per_s", "value": m_plus / (ms_per_step * 1e-3),
         "unit": "oriented edges/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -299,26 +430,73 @@ def main():
         "config": {"workload": wname, "n": st["n"], "lines": st["m_in"],
                    "m_oriented": m_plus, "dispatch": args.dispatch,
                    "parallelism": "single GPU"},
+        "timed_region": TIMED_REGION,
         "counts": {"balanced": counts[0], "unbalanced": counts[1], "exact_vs_reference": exact},
         "ms_count_event_median": round(st["ms_count"], 4), "ms_bin": round(st["ms_bin"], 4),
         "ms_kernels": dict(zip(KERNELS, [round(x, 4) for x in st["ms_kernel"]])),
         "build_ms": round(st0["ms_build"], 3), "upload_ms": round(st0["ms_upload"], 3),
-        "b_alg_bytes": st["b_alg"],
-        "roofline": roofline,
+        "roofline": roofline_of(st, wname),
         "edges_by_strategy": {"merge": st["edges_merge"], "hash": st["edges_hash"],
                               "bitmap": st["edges_bitmap"], "pull": st["edges_pull"],
                               "skipped": st["edges_skipped"]},
         "model_loaded": ctx.model_status == 0,
     }
-    if not args.no_hub:
-        line["learned_vs_fixed"] = hub_learned_vs_fixed(ctx, stc, max(3, args.steps // 2))
-    if not args.no_batch:
-        line["batch"] = batch_workload(ctx, stc, 3)
+    configs = []
+    graphs = {3: g}
+    if not args.no_configs:
+        for wid in (1, 2, 3, 4):
+            gw = graphs.get(wid) or stc.gen_workload(wid)
+            graphs[wid] = gw
+            rec = config_record(ctx, wid, gw, args.reps, 2)
+            configs.append(rec)
+            print("config %s: %.4f ms, learned/fixed %.3f" % (rec["workload"], rec["t_count_ms"],
+                                                            rec["learned_vs_fixed"]),
+                  file=sys.stderr, flush=True)
+        batch = stc.gen_workload_batch(4096)
+        rec = config_record(ctx, 5, batch, 5, 0, batch=True)
+        configs.append(rec)
+        print("config batch4096: %.4f ms" % rec["t_count_ms"], file=sys.stderr, flush=True)
+        hub = configs[3]
+        line["learned_vs_fixed"] = {
+            "workload": hub["workload"], "fixed_ms": hub["ms_by_dispatch"]["fixed"],
+            "learned_ms": hub["ms_by_dispatch"]["learned"], "speedup": hub["learned_vs_fixed"],
+            "learned_vs_best_forced": hub["learned_vs_best_forced"],
+            "best_forced": hub["best_forced"],
+            "exact_equal": True,  # config_record raises when dispatch modes disagree
+            "fixed_rule": "tuned (models/tune.txt; autotune.py on the dispatcher's training set)"}
     ctx.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+        csr, line["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+        if configs:
+            for rec, wid in zip(configs, (1, 2, 3, 4)):
+                if wid == 3:
+                    continue  # the headline sample above covers RMAT-22's merge path
+                rec["cpu"] = cpu_paths([graphs[wid]])
+            # RMAT-22: one full hash count beside the headline merge sample
+            threads, nproc = cpu_threads()
+            th, tb, _, _ = csr.time_hash(threads)
+            cb = line["cpu_baseline"]
+            configs[2]["cpu"] = {"threads": threads, "nproc": nproc,
+                                 "merge_edges_per_s": cb["value"],
+                                 "merge_s": round(m_plus / cb["value"], 3) if cb["value"] else None,
+                                 "hash_s": round(th, 3), "hash_table_build_s": round(tb, 3),
+                                 "hash_edges_per_s": m_plus / th if th > 0 else None,
+                                 "note": "merge: the headline cpu_baseline sample; hash: one run"}
+            # batch: every 16th graph (the sample the reference goldens pin)
+            sample = [batch.graph(i) for i in range(0, batch.k, 16)]
+            cpu = cpu_paths(sample)
+            cpu["sample"] = "every 16th graph: %d of %d" % (len(sample), batch.k)
+            configs[4]["cpu"] = cpu
+            for rec in configs:
+                c = rec["cpu"]
+                if c.get("merge_edges_per_s"):
+                    c["gpu_over_cpu_merge"] = round(rec["oriented_edges_per_s"] /
+                                                    c["merge_edges_per_s"], 1)
+        del csr
     else:
         line["cpu_baseline"] = None
+    if configs:
+        line["configs"] = configs
     print(json.dumps(line), flush=True)
 
 

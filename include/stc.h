@@ -112,7 +112,9 @@ typedef struct stc_stats {
                               library stream), median over repeats; a
                               count with workgroup-pull edges runs both pull
                               slots in one launch, reported in the last slot
-                              (wave-pull then reads about 0) */
+                              (wave-pull then reads about 0); a batch counts
+                              every item in one launch, reported in the last
+                              slot (timed on a warm eager run when repeat > 1) */
   double bytes_kernel[8];  /* merge-model bytes 4(d+(u)+d+(v)) of the
                               oriented edges each of those kernels counted
                               (the same pull folding) */

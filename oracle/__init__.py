@@ -63,6 +63,9 @@ def lib():
         L.orc_count_merge_sample.restype = C.c_double
         L.orc_time_merge.argtypes = [P(_Csr), C.c_int, P(C.c_uint64), P(C.c_uint64)]
         L.orc_time_merge.restype = C.c_double
+        L.orc_time_hash.argtypes = [P(_Csr), C.c_int, P(C.c_double), P(C.c_uint64),
+                                    P(C.c_uint64)]
+        L.orc_time_hash.restype = C.c_double
         _L = L
     return _L
 
@@ -130,6 +133,13 @@ class Csr:
         b, u = C.c_uint64(), C.c_uint64()
         t = lib().orc_time_merge(self._p, threads, C.byref(b), C.byref(u))
         return t, int(b.value), int(u.value)
+
+    def time_hash(self, threads=0):
+        """(count seconds, table build seconds, balanced, unbalanced) of the
+        hash path."""
+        bs, b, u = C.c_double(), C.c_uint64(), C.c_uint64()
+        t = lib().orc_time_hash(self._p, threads, C.byref(bs), C.byref(b), C.byref(u))
+        return t, float(bs.value), int(b.value), int(u.value)
 
     def merge_sample(self, stride, phase=0, threads=0):
         ne, by, b, u = C.c_uint64(), C.c_double(), C.c_uint64(), C.c_uint64()

@@ -337,9 +337,13 @@ static inline uint32_t orc_next_pow2(uint32_t x) {
   return x + 1;
 }
 
-void orc_count_hash(const orc_csr *c, int threads, uint64_t *bal,
-                    uint64_t *unbal) {
+/* The hash path in its two timed phases: the per-vertex tables (the
+ * reference's "Build Hash Tables" timer, tc_hash_sign_enu.cpp:100-122) and the
+ * count (its "Triangle Counting Execution", :124-165); wall seconds of each. */
+static void hash_phases(const orc_csr *c, int threads, uint64_t *bal, uint64_t *unbal,
+                        double *build_s, double *count_s) {
   if (threads > 0) omp_set_num_threads(threads);
+  const double t0 = omp_get_wtime();
   const uint32_t n = c->n;
   uint64_t *ho = (uint64_t *)malloc(((uint64_t)n + 1) * 8);
   ho[0] = 0;
@@ -357,6 +361,7 @@ void orc_count_hash(const orc_csr *c, int threads, uint64_t *bal,
       t[pos].sign = c->signs[e];
     }
   }
+  const double t1 = omp_get_wtime();
   const uint32_t *R = c->rank_;
   unsigned long long B = 0, U = 0;
   #pragma omp parallel for schedule(dynamic, 16) reduction(+ : B, U)
@@ -384,9 +389,24 @@ void orc_count_hash(const orc_csr *c, int threads, uint64_t *bal,
       }
     }
   }
+  const double t2 = omp_get_wtime();
   free(arena);
   free(ho);
   *bal = B; *unbal = U;
+  if (build_s) *build_s = t1 - t0;
+  if (count_s) *count_s = t2 - t1;
+}
+
+void orc_count_hash(const orc_csr *c, int threads, uint64_t *bal,
+                    uint64_t *unbal) {
+  hash_phases(c, threads, bal, unbal, NULL, NULL);
+}
+
+double orc_time_hash(const orc_csr *c, int threads, double *build_s, uint64_t *bal,
+                     uint64_t *unbal) {
+  double count_s = 0;
+  hash_phases(c, threads, bal, unbal, build_s, &count_s);
+  return count_s;
 }
 
 /* ---- four classes: tc_serial.cpp:95-170 ---------------------------------- */

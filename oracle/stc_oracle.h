@@ -70,6 +70,10 @@ double orc_count_merge_sample(const orc_csr *c, int threads, uint32_t stride,
  * "Triangle Counting Execution" phase, tc_merge_sign_enu.cpp:131-145). */
 double orc_time_merge(const orc_csr *c, int threads, uint64_t *bal,
                       uint64_t *unbal);
+/* Wall seconds of the hash path's counting loop (tc_hash_sign_enu.cpp:124-165);
+ * *build_s gets the seconds of its per-vertex table build (:100-122). */
+double orc_time_hash(const orc_csr *c, int threads, double *build_s, uint64_t *bal,
+                     uint64_t *unbal);
 
 #ifdef __cplusplus
 }

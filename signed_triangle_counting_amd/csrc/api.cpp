@@ -49,6 +49,8 @@ struct stc_ctx {
   UploadRing ring;  // pinned staging for host edge lists (allocated on first use)
   bool fused_pull = false;  // last count was one graph's: with workgroup-pull edges, one
                             // launch counted both pull slots (its time in the last)
+  bool batch_launch = false;  // last count was a batch's: one launch counted every
+                              // item (its time, edges' bytes in the last slot)
   // Device address of the library's static scratch array on this context's
   // device (hipGetSymbolAddress at stc_create); null with scratch_err set when
   // that call failed.
@@ -303,6 +305,14 @@ float median(std::vector<float> v) {
   return (k & 1) ? v[k / 2] : 0.5f * (v[k / 2 - 1] + v[k / 2]);
 }
 
+// Binning time and per-kernel times of the run whose marks were recorded.
+hipError_t read_marks(stc_ctx *c, float &ms_bin, float kt[kKernels]) {
+  hipError_t e = hipEventElapsedTime(&ms_bin, c->marks[0], c->marks[1]);
+  for (int k = 0; k < kKernels && e == hipSuccess; k++)
+    e = hipEventElapsedTime(&kt[k], c->marks[k + 1], c->marks[k + 2]);
+  return e;
+}
+
 // The device table (Strategy values) as STC_TABLE_* codes.
 void keep_table(stc_ctx *c, const CountResult &res) {
   for (int b = 0; b < kBuckets; b++) {
@@ -518,6 +528,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
     out->unbalanced = res.t[1];
   }
   c->fused_pull = true;
+  c->batch_launch = false;
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
   c->stats.ms_bin = bins.back();
@@ -614,7 +625,13 @@ int stc_get_stats(stc_ctx *c, stc_stats *out) {
       c->stats.bytes_kernel[k] = 4.0 * (double)res.kern_words[k];
       c->stats.alg_bytes_kernel[k] = (double)res.kern_bytes[k];
     }
-    if (c->fused_pull && res.kern_edges[7] > 0) {  // one launch counted both pull slots
+    if (c->batch_launch) {  // one launch counted every item of the batch
+      for (int k = 0; k < kKernels - 1; k++) {
+        c->stats.bytes_kernel[kKernels - 1] += c->stats.bytes_kernel[k];
+        c->stats.alg_bytes_kernel[kKernels - 1] += c->stats.alg_bytes_kernel[k];
+        c->stats.bytes_kernel[k] = c->stats.alg_bytes_kernel[k] = 0.0;
+      }
+    } else if (c->fused_pull && res.kern_edges[7] > 0) {  // one launch counted both pull slots
       c->stats.bytes_kernel[7] += c->stats.bytes_kernel[6];
       c->stats.bytes_kernel[6] = 0.0;
       c->stats.alg_bytes_kernel[7] += c->stats.alg_bytes_kernel[6];
@@ -770,16 +787,20 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
   const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
-  std::vector<float> times;
+  std::vector<float> times, bins, kt[kKernels];
   for (int r = 0; r < repeat && e == hipSuccess; r++) {
     e = hipEventRecord(c->ev[0], c->st);
     if (e == hipSuccess)
-      e = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st);
+      e = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st,
+                        c->marks);
     if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
     if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
-    float ms = 0.f;
+    float ms = 0.f, msb = 0.f, t[kKernels] = {};
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    if (e == hipSuccess) e = read_marks(c, msb, t);
     times.push_back(ms);
+    bins.push_back(msb);
+    for (int j = 0; j < kKernels; j++) kt[j].push_back(t[j]);
   }
   std::vector<unsigned long long> got(2ull * k);
   if (e == hipSuccess) e = hipMemcpyAsync(got.data(), d_pg, 16ull * k, hipMemcpyDeviceToHost, c->st);
@@ -794,7 +815,10 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
   }
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->stats.ms_bin = median(bins);
+  for (int j = 0; j < kKernels; j++) c->stats.ms_kernel[j] = median(kt[j]);
   c->fused_pull = false;
+  c->batch_launch = true;
   c->kstats_pending = true;
   if (res_out) *res_out = res;
   return STC_OK;
@@ -855,18 +879,25 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
   hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
   const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
-  if (e == hipSuccess)
-    e = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st);
+  // eager run (allocates every buffer; its counts are what each replay must
+  // give), then with repeats a second, warm eager run whose event marks give
+  // the binning and batched-launch times
+  float msb = 0.f, kt[kKernels] = {};
   std::vector<unsigned long long> eager(2ull * k), got(2ull * k);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(eager.data(), d_pg, 16ull * k, hipMemcpyDeviceToHost, c->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  for (int r = 0; r < (repeat > 1 ? 2 : 1) && e == hipSuccess; r++) {
+    e = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st, c->marks);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(eager.data(), d_pg, 16ull * k, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = read_marks(c, msb, kt);
+  }
   if (e != hipSuccess) return hip_fail(c, e, "batch count");
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   e = hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed);
   if (e == hipSuccess) {
-    hipError_t e2 = enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st);
+    hipError_t e2 =
+        enqueue_batch(c->g, c->hp, c->dp, c->cb, graph_of, d_pg, k, c->num_cus, c->st, nullptr);
     e = hipStreamEndCapture(c->st, &graph);
     if (e2 != hipSuccess) e = e2;
   }
@@ -901,7 +932,10 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
   }
   c->stats.ms_count = median(times);
   c->stats.ms_count_min = *std::min_element(times.begin(), times.end());
+  c->stats.ms_bin = msb;
+  for (int j = 0; j < kKernels; j++) c->stats.ms_kernel[j] = kt[j];
   c->fused_pull = false;
+  c->batch_launch = true;
   c->kstats_pending = true;
   c->stats.dispatch_used = (dispatch == STC_LEARNED && !c->hp.have_model) ? STC_FIXED : dispatch;
   c->have_classes = false;

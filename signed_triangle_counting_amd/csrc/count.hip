@@ -2504,14 +2504,18 @@ hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const Dis
 hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                          CountBuffers &cb, const uint32_t *graph_of,
                          unsigned long long *per_graph, uint32_t k, int num_cus,
-                         hipStream_t st) {
+                         hipStream_t st, const hipEvent_t *marks) {
   BinOut L;
-  STC_TRY(enqueue_binning(g, dp, cb, num_cus, st, nullptr, L));
+  STC_TRY(enqueue_binning(g, dp, cb, num_cus, st, marks, L));
   hipLaunchKernelGGL(k_fill_words, dim3(grid_for(4ull * k, 256, 4096)), dim3(256), 0, st,
                      (uint32_t *)per_graph, 4ull * k, 0u);
   STC_TRY(hipGetLastError());
-  return launch_two_phase(g, hp, dp, cb, L, graph_of, per_graph, num_cus, st, 3u, 4u, false,
-                          false);
+  // one launch counts every item: its time is the last kernel slot's
+  for (int i = 2; marks && i < kMarks - 1; i++) STC_TRY(hipEventRecord(marks[i], st));
+  STC_TRY(launch_two_phase(g, hp, dp, cb, L, graph_of, per_graph, num_cus, st, 3u, 4u, false,
+                           false));
+  if (marks) STC_TRY(hipEventRecord(marks[kMarks - 1], st));
+  return hipSuccess;
 }
 
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_t st) {

@@ -288,10 +288,12 @@ size_t wave_bitmap_lds_bytes(const TuneParams &t);
 // Batched counting over the union CSR of k graphs: binning, then one
 // persistent kernel; per_graph gets k (balanced, unbalanced) pairs.
 // graph_of maps a rank to its graph.
+// marks (nullable; never during stream capture): as enqueue_count's, with
+// the single batched launch between the last two.
 hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const DispatchParams *dp,
                          CountBuffers &cb, const uint32_t *graph_of,
                          unsigned long long *per_graph, uint32_t k, int num_cus,
-                         hipStream_t st);
+                         hipStream_t st, const hipEvent_t *marks);
 size_t batch_lds_bytes(const TuneParams &t);
 hipError_t init_kernel_attributes();
 
