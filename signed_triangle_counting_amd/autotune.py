@@ -118,8 +118,7 @@ class Timer:
 
     def time(self, name, p, mode, batch=False):
         self.ctx.set_tune(tune_text(p))
-        if batch:
-            self.ctx.count_batch_prepared(dispatch=mode, repeat=1)
+        if batch:  # the call's warm eager run precedes the timed replays
             got = self.ctx.count_batch_prepared(dispatch=mode, repeat=self.reps)
             got = (sum(x[0] for x in got), sum(x[1] for x in got))
         else:
@@ -198,6 +197,8 @@ def tune_fixed(ctx, timer, best, log):
             sig = fixed_table(hist, f)
             if sig not in sig_time:
                 sig_time[sig] = timer.time(name, dict(best, **f), "fixed", is_batch)
+                if len(sig_time) % 100 == 0:
+                    log("    %s: %d tables timed" % (name, len(sig_time)))
             times.append(sig_time[sig])
         per_graph.append({"graph": name, "best_forced_ms": best_forced, "forced_ms": forced,
                           "times": times, "distinct_tables": len(sig_time)})

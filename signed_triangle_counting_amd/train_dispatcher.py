@@ -52,6 +52,13 @@ def training_graphs(quick=False):
         ("hub_4m", lambda: stc.gen_hub(4000000, 4300000, 4, 1.5, 0.7, 40000, 0.1, 114)),
         ("rmat22_ef16", lambda: stc.gen_rmat(22, 16, 0.57, 0.19, 0.19, 0.25, 115)),
         ("er_2e6_2e7", lambda: stc.gen_er(2000000, 20000000, 0.3, 116)),
+        # the benchmark configurations' sizes, other seeds: a dispatcher that
+        # has seen no graph this small (ER) or this large (hub) extrapolates
+        ("er_1e4_1e5", lambda: stc.gen_er(10000, 100000, 0.3, 117)),
+        ("er_3e4_4e5", lambda: stc.gen_er(30000, 400000, 0.3, 118)),
+        ("grid_960", lambda: stc.gen_grid(960, 960, 0.5, 0.2, 119)),
+        ("hub_18m", lambda: stc.gen_hub(18000000, 19000000, 8, 1.2, 0.6, 200000, 0.1, 120)),
+        ("hub_12m", lambda: stc.gen_hub(12000000, 13000000, 6, 1.1, 0.5, 150000, 0.1, 124)),
     ]
     return g[:4] + g[6:7] + g[11:12] if quick else g
 
@@ -63,6 +70,7 @@ def training_batches(quick=False):
         ("batch_512", lambda: stc.gen_batch(512, seed=121)),
         ("batch_256_dense", lambda: stc.gen_batch(256, d_lo=16.0, d_hi=64.0, seed=122)),
         ("batch_1024_small", lambda: stc.gen_batch(1024, n_hi=8192, seed=123)),
+        ("batch_2048", lambda: stc.gen_batch(2048, n_hi=16384, seed=125)),
     ]
     return b[:1] if quick else b
 
@@ -104,7 +112,9 @@ def collect(ctx, name, g, reps=3, log=print, batch=False):
     table[nz] = stc.TABLE[best_s]
     order = sorted(nz, key=lambda b: -float(hist[b]) * 2.0 ** ((b // 16) + (b % 16)))
     feats, costs = [], []
-    for b in order:
+    for i, b in enumerate(order):
+        if i and i % 25 == 0:
+            log("    %s: %d of %d buckets" % (name, i, len(order)))
         c = []
         for s in STRATS:
             t = table.copy()

@@ -394,11 +394,14 @@ def test_kernel_statistics_account_for_every_edge(ctx):
     srcs = np.unique(srcr[counted])
     want_bitmap = int((4 * db)[counted].sum()) + int((20 + 16 * fdr[srcs]).sum())
     # pull: 4 da per counted edge; per target with a counted in-edge, chunks
-    # of 2048 (d+ > 64) or 256 in-edges, each 28 + 4 d+(v), and 8 bytes per
-    # in-edge record
+    # of the shipped tune's workgroup (d+ > 64) or wave chunk size in in-edges,
+    # each 28 + 4 d+(v), and 8 bytes per in-edge record
+    from signed_triangle_counting_amd import autotune
+    tune = autotune.read_tune(os.path.join(stc.MODEL_DIR, "tune.txt"))
     tg = np.unique(dstr[counted])
     indeg = np.bincount(dstr, minlength=n)[tg]
-    chunk = np.where(fdr[tg] > 64, 2048, 256)
+    chunk = np.where(fdr[tg] > 64, 1 << tune["pull_block_chunk_log2"],
+                     1 << tune["pull_wave_chunk_log2"])
     nchunks = (indeg + chunk - 1) // chunk
     want_pull = (int((4 * da)[counted].sum()) + int((nchunks * (28 + 4 * fdr[tg])).sum()) +
                  int((8 * indeg).sum()))
