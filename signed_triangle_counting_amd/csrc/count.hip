@@ -1312,6 +1312,109 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
   }
 }
 
+// Row-packed streaming: the kept lists of a 64-edge metadata batch are cut
+// into rows of 64 consecutive entries (list i holds ceil(len_i / 64) rows,
+// the rows of the batch numbered by an inclusive scan), and a unit is the
+// next kRows rows whatever lists they come from.  Each row is one buffer load
+// through its own descriptor, built when the load is issued, whose range ends
+// at its list's end, so only the last row of a list is part-filled: short
+// lists (a workgroup-pull suffix is about 108 entries) no longer leave most of
+// a unit's loads empty, and a wave keeps two full units of loads in flight.
+// Lanes 0 .. kRows - 1 locate a unit's rows together: row r belongs to the
+// first lane whose inclusive row count exceeds r (a binary search over the
+// scan with lane shuffles); each row's triangles are added at once with its
+// edge's sign, so no per-list state crosses rows.
+constexpr uint32_t kRows = kUnroll;
+
+template <bool CLS, class Meta, class Keep, class Probe>
+__device__ __forceinline__ void stream_rows(const DevCsr &g, uint32_t first, uint32_t ea,
+                                            uint32_t step, const Meta &meta, const Keep &keep,
+                                            const Probe &probe, Acc<CLS> &acc) {
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  auto lane_of = [](uint32_t v, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)v, i); };
+  auto shfl = [](uint32_t v, uint32_t i) { return (uint32_t)__shfl((int)v, (int)i, 64); };
+  for (uint32_t j0 = first; j0 < ea; j0 += 64u * step) {
+    const uint32_t j = j0 + lane * step;
+    uint32_t w = 0, pb = 0, eb = 0, x = 0;
+    if (j < ea) meta(j, w, pb, eb, x);
+    const bool kept = j < ea && keep(j, pb, eb);
+    const uint32_t rows = kept ? (eb - pb + 63u) >> 6 : 0u;
+    const uint32_t incl = wave_scan_incl(rows);
+    const uint32_t total = lane_of(incl, 63);
+    if (total == 0u) continue;
+    // Per lane u < kRows, row R + u of the batch: its first entry and its
+    // byte count (0 past the last row, which then reads nothing); the rows'
+    // edge signs as a bit mask.
+    auto locate = [&](uint32_t R, uint32_t &posv, uint32_t &nrecv) -> uint32_t {
+      const uint32_t r = R + (lane & (kRows - 1u));
+      uint32_t o = 0;  // lanes before the owner: incl <= r
+#pragma unroll
+      for (uint32_t st = 32; st > 0; st >>= 1)
+        if (shfl(incl, o + st - 1u) <= r) o += st;
+      o = min(o, 63u);
+      const uint32_t ro = shfl(incl, o) - shfl(rows, o);  // the owner's first row
+      const uint32_t p = shfl(pb, o) + 64u * (r - ro);
+      const uint32_t e = shfl(eb, o);
+      // every lane takes part in the shuffles: a lane shuffle reads nothing
+      // from a lane that is switched off at that point
+      const uint32_t sg = shfl(w, o) & 1u;
+      const bool live = r < total;
+      posv = live ? p : 0u;
+      nrecv = live ? min(e - p, 64u) * 4u : 0u;
+      return (uint32_t)__ballot(lane < kRows && live && sg);
+    };
+    auto load = [&](uint32_t(&y)[kRows], uint32_t R) -> uint32_t {
+      uint32_t posv, nrecv;
+      const uint32_t sabm = locate(R, posv, nrecv);
+#pragma unroll
+      for (uint32_t u = 0; u < kRows; u++) {
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(col + lane_of(posv, (int)u)), (short)0, (int)lane_of(nrecv, (int)u),
+            kBufferWord3);
+        y[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(lane * 4u), 0, 0);
+      }
+      return sabm;
+    };
+    auto probe_unit = [&](const uint32_t(&y)[kRows], uint32_t R, uint32_t sabm) {
+#pragma unroll
+      for (uint32_t u = 0; u < kRows; u++) {
+        if (R + u >= total) break;  // wave-uniform
+        bool same, diff;
+        probe(y[u], same, diff);  // lanes past the row's end read 0, which misses
+        uint32_t e0 = 0, e1 = 0, e2 = 0;
+        tally<CLS>(same, diff, y[u] & 1u, e0, e1, e2);
+        acc.add_edge((sabm >> u) & 1u, e0, e1, e2);
+      }
+      // retire the unit's loads here, so none is left pending past its use
+#pragma unroll
+      for (uint32_t u = 0; u < kRows; u++) asm volatile("" ::"v"(y[u]));
+    };
+    uint32_t ya[kRows], yb[kRows];
+    uint32_t R = 0;
+    uint32_t sa = load(ya, R);
+    // two units in turn: one's loads are in flight while the other is probed
+    while (true) {
+      const uint32_t sb = load(yb, R + kRows);
+      probe_unit(ya, R, sa);
+      if (R + kRows >= total) {
+#pragma unroll
+        for (uint32_t u = 0; u < kRows; u++) asm volatile("" ::"v"(yb[u]));
+        break;
+      }
+      R += kRows;
+      sa = load(ya, R + kRows);
+      probe_unit(yb, R, sb);
+      if (R + kRows >= total) {
+#pragma unroll
+        for (uint32_t u = 0; u < kRows; u++) asm volatile("" ::"v"(ya[u]));
+        break;
+      }
+      R += kRows;
+    }
+  }
+}
+
 // Inclusive max-scan over the wave.
 __device__ __forceinline__ uint32_t wave_max_scan_incl(uint32_t x) {
   const uint32_t lane = __lane_id();
@@ -1716,10 +1819,12 @@ __device__ __forceinline__ void pull_stream(const DevCsr &g, uint32_t first, uin
   auto keep = [&](uint32_t, uint32_t pb, uint32_t eb) {
     return eb > pb && table[bucket_of(eb - pb, d)] == S_PULL;
   };
-  if (!PACKED || (CLS && g.pv))
+  if (CLS && g.pv)  // per-vertex counts credit each edge's owner
     stream_edges<CLS, true>(g, first, t1, step, PullMeta{g}, keep, probe, acc);
-  else
+  else if (PACKED)
     stream_packed<CLS>(g, first, t1, step, PullMeta{g}, keep, probe, H, acc);
+  else
+    stream_rows<CLS>(g, first, t1, step, PullMeta{g}, keep, probe, acc);
 }
 
 // The chunk of target v's in-edges starting at position j0.
