@@ -64,6 +64,8 @@ def parse_args():
                     help="only the headline RMAT-22 line (no per-config records)")
     ap.add_argument("--profile-config", type=int, default=0,
                     help="count only config N (1-5) with learned dispatch, for rocprofv3")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 1 GiB device-to-device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="measure the HBM read rate for sequential and random-segment reads")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -396,6 +398,14 @@ def main():
     import torch
     import signed_triangle_counting_amd as stc
 
+    if args.copy_calibration:
+        a = torch.full((1 << 28,), 7, dtype=torch.int32, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": 1 << 30, "ok": bool((b[:16] == 7).all())}), flush=True)
+        return
     if args.hbm_probe:
         print(json.dumps({"hbm_read_tbps": hbm_probe(0),
                           "peak_tbps": HBM_PEAK_GBPS / 1000.0}), flush=True)
