@@ -1320,11 +1320,12 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
 // at its list's end, so only the last row of a list is part-filled: short
 // lists (a workgroup-pull suffix is about 108 entries) no longer leave most of
 // a unit's loads empty, and a wave keeps two full units of loads in flight.
-// Lanes 0 .. kRows - 1 locate a unit's rows together: row r belongs to the
-// first lane whose inclusive row count exceeds r (a binary search over the
-// scan with lane shuffles); each row's triangles are added at once with its
-// edge's sign, so no per-list state crosses rows.
+// The lanes locate 64 rows at a time: row r belongs to the first lane whose
+// inclusive row count exceeds r (a binary search over the scan with lane
+// shuffles); each row's triangles are added at once with its edge's sign, so
+// no per-list state crosses rows.
 constexpr uint32_t kRows = kUnroll;
+static_assert(64u % kRows == 0u, "a unit of rows never straddles two groups of 64 rows");
 
 template <bool CLS, class Meta, class Keep, class Probe>
 __device__ __forceinline__ void stream_rows(const DevCsr &g, uint32_t first, uint32_t ea,
@@ -1343,11 +1344,14 @@ __device__ __forceinline__ void stream_rows(const DevCsr &g, uint32_t first, uin
     const uint32_t incl = wave_scan_incl(rows);
     const uint32_t total = lane_of(incl, 63);
     if (total == 0u) continue;
-    // Per lane u < kRows, row R + u of the batch: its first entry and its
-    // byte count (0 past the last row, which then reads nothing); the rows'
-    // edge signs as a bit mask.
-    auto locate = [&](uint32_t R, uint32_t &posv, uint32_t &nrecv) -> uint32_t {
-      const uint32_t r = R + (lane & (kRows - 1u));
+    // Rows are located 64 at a time, one per lane (a unit's rows then come
+    // from two readlanes each): row G + lane's first entry and byte count (0
+    // past the last row, which then reads nothing), and the rows' edge
+    // signs as a bit mask.
+    uint32_t posg = 0, nrecg = 0;
+    uint64_t sgg = 0;
+    auto group = [&](uint32_t G) {
+      const uint32_t r = G + lane;
       uint32_t o = 0;  // lanes before the owner: incl <= r
 #pragma unroll
       for (uint32_t st = 32; st > 0; st >>= 1)
@@ -1360,21 +1364,21 @@ __device__ __forceinline__ void stream_rows(const DevCsr &g, uint32_t first, uin
       // from a lane that is switched off at that point
       const uint32_t sg = shfl(w, o) & 1u;
       const bool live = r < total;
-      posv = live ? p : 0u;
-      nrecv = live ? min(e - p, 64u) * 4u : 0u;
-      return (uint32_t)__ballot(lane < kRows && live && sg);
+      posg = live ? p : 0u;
+      nrecg = live ? min(e - p, 64u) * 4u : 0u;
+      sgg = __ballot(live && sg);
     };
     auto load = [&](uint32_t(&y)[kRows], uint32_t R) -> uint32_t {
-      uint32_t posv, nrecv;
-      const uint32_t sabm = locate(R, posv, nrecv);
+      if ((R & 63u) == 0u) group(R);  // wave-uniform
+      const uint32_t l0 = R & 63u;
 #pragma unroll
       for (uint32_t u = 0; u < kRows; u++) {
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(col + lane_of(posv, (int)u)), (short)0, (int)lane_of(nrecv, (int)u),
-            kBufferWord3);
+            (void *)(col + lane_of(posg, (int)(l0 + u))), (short)0,
+            (int)lane_of(nrecg, (int)(l0 + u)), kBufferWord3);
         y[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(lane * 4u), 0, 0);
       }
-      return sabm;
+      return (uint32_t)(sgg >> l0) & ((1u << kRows) - 1u);
     };
     auto probe_unit = [&](const uint32_t(&y)[kRows], uint32_t R, uint32_t sabm) {
 #pragma unroll
