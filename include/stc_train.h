@@ -69,6 +69,19 @@ int stc_train_dispatcher(stc_ctx *ctx, const float *feat, const float *cost, int
  * signed_tc.hxx:47,65). */
 int stc_last_table(const stc_ctx *ctx, uint8_t out[STC_BUCKETS]);
 
+/* The prepared graph's (or batch's) device CSR in rank space, copied to the
+ * host (every pointer nullable; n and m from stc_get_stats): off[n + 1] the
+ * out-list offsets by rank; col[m] the lists, (rank << 1) | neg ascending by
+ * rank; perm[n] the vertex id of each rank; tin_off[n + 1] and tin[2 m] the
+ * transposed index, per target rank its in-edges e = (u -> v) in ascending
+ * e as (e + 1, (|N+(u) after v| << 1) | neg) pairs; dcls[n] each rank's
+ * out-degree class min(ceil_log2 d+, 15) (0xFF for an empty list); smask[n]
+ * the suffix classes of its in-edges.  Replaces, for tests, the reference's
+ * ForwardCSR (Multicore Implementations/graph_io.h:22-30,221-402), which
+ * orients by the same (degree, id) rank but keeps vertex ids. */
+int stc_get_csr(stc_ctx *ctx, uint32_t *off, uint32_t *col, uint32_t *perm, uint32_t *tin_off,
+                uint32_t *tin, uint8_t *dcls, uint32_t *smask);
+
 /* Reload "dispatch_mlp.txt" / "tune.txt" from dir (NULL: the shipped ones).
  * A directory without a valid tune.txt keeps the context's tune values. */
 int stc_load_models(stc_ctx *ctx, const char *dir);

@@ -47,6 +47,7 @@ EXPORTED = [
     "stc_batch_free", "stc_gen_workload", "stc_gen_workload_batch",
     "stc_bucket_histogram", "stc_count_with_table", "stc_bucket_features",
     "stc_train_dispatcher", "stc_load_models", "stc_set_tune", "stc_last_table",
+    "stc_get_csr",
 ]
 
 TABLE = {"skip": 0, "merge": 1, "hash": 2, "bitmap": 3, "pull": 4}
@@ -170,6 +171,7 @@ def lib():
     L.stc_load_models.argtypes = [C.c_void_p, C.c_char_p]
     L.stc_set_tune.argtypes = [C.c_void_p, C.c_char_p]
     L.stc_last_table.argtypes = [C.c_void_p, C.c_void_p]
+    L.stc_get_csr.argtypes = [C.c_void_p] + [C.c_void_p] * 7
     _lib_handle = L
     return L
 
@@ -439,6 +441,23 @@ class Counter:
         out = np.zeros(256, np.uint8)
         _check(lib().stc_last_table(self._h, out.ctypes.data), self._h)
         return out
+
+    def csr_arrays(self):
+        """The prepared graph's device CSR in rank space (stc_get_csr): off,
+        col ((rank << 1) | neg), perm (rank -> vertex id), tin_off, tin ((m, 2):
+        suffix start, (suffix length << 1) | neg), dcls, smask."""
+        st = self.stats()
+        n, m = st["n"], st["m_oriented"]
+        a = {"off": np.zeros(n + 1, np.uint32), "col": np.zeros(max(m, 1), np.uint32),
+             "perm": np.zeros(max(n, 1), np.uint32), "tin_off": np.zeros(n + 1, np.uint32),
+             "tin": np.zeros((max(m, 1), 2), np.uint32), "dcls": np.zeros(max(n, 1), np.uint8),
+             "smask": np.zeros(max(n, 1), np.uint32)}
+        _check(lib().stc_get_csr(self._h, *[a[k].ctypes.data for k in
+                                            ("off", "col", "perm", "tin_off", "tin", "dcls",
+                                             "smask")]), self._h)
+        a["col"], a["tin"] = a["col"][:m], a["tin"][:m]
+        a["perm"], a["dcls"], a["smask"] = a["perm"][:n], a["dcls"][:n], a["smask"][:n]
+        return a
 
     def load_models(self, model_dir=None):
         rc = lib().stc_load_models(self._h, model_dir.encode() if model_dir else None)

@@ -1056,6 +1056,29 @@ extern "C" int stc_train_dispatcher(stc_ctx *c, const float *feat, const float *
   return STC_OK;
 }
 
+extern "C" int stc_get_csr(stc_ctx *c, uint32_t *off, uint32_t *col, uint32_t *perm,
+                           uint32_t *tin_off, uint32_t *tin, uint8_t *dcls, uint32_t *smask) {
+  if (!c) return STC_E_ARG;
+  if (!c->prepared && !c->batch_ready) return fail(c, STC_E_ARG, "nothing prepared");
+  (void)hipSetDevice(c->device);
+  const uint64_t n = c->g.n, m = c->g.m;
+  struct Part {
+    void *dst;
+    const void *src;
+    uint64_t bytes;
+  } parts[] = {{off, c->g.off, (n + 1) * 4},        {col, c->g.col, m * 4},
+               {perm, c->bb.perm.p, n * 4},         {tin_off, c->g.tin_off, (n + 1) * 4},
+               {tin, c->g.tin, m * 8},              {dcls, c->g.dcls, n},
+               {smask, c->g.smask, n * 4}};
+  hipError_t e = hipSuccess;
+  for (const Part &pt : parts)
+    if (pt.dst && pt.bytes && e == hipSuccess)
+      e = hipMemcpyAsync(pt.dst, pt.src, pt.bytes, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "csr readback");
+  return STC_OK;
+}
+
 extern "C" int stc_last_table(const stc_ctx *c, uint8_t out[STC_BUCKETS]) {
   if (!c || !out) return STC_E_ARG;
   if (!c->have_table) return STC_E_ARG;
