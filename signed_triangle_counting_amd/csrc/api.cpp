@@ -26,7 +26,8 @@ using namespace stc;
 
 struct stc_ctx {
   int device = 0;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;  // every kernel and the small copies
+  hipStream_t up = nullptr;  // staged host -> device copies of edge lists
   int num_cus = 256;
   std::string err;
   int model_status = STC_OK;
@@ -355,6 +356,7 @@ int stc_create(int device, const char *model_dir, stc_ctx **out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       c->num_cus = prop.multiProcessorCount;
     e = hipStreamCreate(&c->st);
+    if (e == hipSuccess) e = hipStreamCreate(&c->up);
   }
   for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
   for (int i = 0; i < kMarks && e == hipSuccess; i++) e = hipEventCreate(&c->marks[i]);
@@ -393,6 +395,7 @@ int stc_create(int device, const char *model_dir, stc_ctx **out) {
 void stc_destroy(stc_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  if (c->up) (void)hipStreamSynchronize(c->up);
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (auto &ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -400,6 +403,7 @@ void stc_destroy(stc_ctx *c) {
     if (ev) (void)hipEventDestroy(ev);
   if (c->dp) (void)hipFree(c->dp);
   c->ring.release();
+  if (c->up) (void)hipStreamDestroy(c->up);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -414,23 +418,43 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   (void)hipSetDevice(c->device);
   c->prepared = false;
   c->batch_ready = false;
-  const uint32_t *du = g->u, *dv = g->v;
-  const int8_t *ds = g->sign;
+  BuildInput in;
+  in.u = g->u;
+  in.v = g->v;
+  in.s = g->sign;
+  in.m_in = g->m;
+  in.n = g->n;
   hipError_t e = hipSuccess;
+  std::string why;
   float ms_up = 0.f;
+  bool canon_done = false;  // lines canonicalised chunk by chunk during the upload
   if (!g_is_device) {
     const uint64_t mm = g->m ? g->m : 1;
     if ((e = c->bb.in_u.ensure(mm * 4)) != hipSuccess ||
         (e = c->bb.in_v.ensure(mm * 4)) != hipSuccess ||
         (e = c->bb.in_s.ensure(mm)) != hipSuccess)
       return hip_fail(c, e, "upload alloc");
+    in.u = c->bb.in_u.as<uint32_t>();
+    in.v = c->bb.in_v.as<uint32_t>();
+    in.s = c->bb.in_s.as<int8_t>();
+  }
+  e = build_begin(in, c->bb, why, c->st);
+  if (e == hipErrorInvalidValue && !why.empty()) return fail(c, STC_E_ARG, why);
+  if (e != hipSuccess) return hip_fail(c, e, "csr build");
+  if (!g_is_device) {
     const bool staged = use_staging(c, g->m);
     e = hipStreamSynchronize(c->st);
     const auto t0 = std::chrono::steady_clock::now();
     if (staged) {
+      // copies on the copy stream; each chunk is canonicalised (and its first
+      // sort digit counted) on the compute stream while the next one copies
       if (e == hipSuccess)
         e = c->ring.upload({EdgePiece{g->u, g->v, g->sign, g->m}}, c->bb.in_u.as<uint32_t>(),
-                           c->bb.in_v.as<uint32_t>(), c->bb.in_s.as<int8_t>(), c->st);
+                           c->bb.in_v.as<uint32_t>(), c->bb.in_s.as<int8_t>(), c->up, c->st,
+                           [&](uint64_t l0, uint64_t l1) {
+                             return build_canon(in, l0, l1, c->bb, c->st);
+                           });
+      canon_done = true;
     } else if (g->m) {
       if (e == hipSuccess)
         e = hipMemcpyAsync(c->bb.in_u.p, g->u, g->m * 4, hipMemcpyHostToDevice, c->st);
@@ -442,14 +466,12 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
     }
     ms_up = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (e != hipSuccess) return hip_fail(c, e, "upload");
-    du = c->bb.in_u.as<uint32_t>();
-    dv = c->bb.in_v.as<uint32_t>();
-    ds = c->bb.in_s.as<int8_t>();
   }
-  std::string why;
   e = hipEventRecord(c->ev[0], c->st);
-  if (e == hipSuccess) e = build_csr(du, dv, ds, g->m, g->n, nullptr, 0, c->bb, c->br, why, c->st);
+  if (e == hipSuccess && !canon_done) e = build_canon(in, 0, in.m_in, c->bb, c->st);
+  if (e == hipSuccess) e = build_finish(in, c->bb, c->br, why, c->st);
   if (e == hipErrorInvalidValue && !why.empty()) return fail(c, STC_E_ARG, why);
+  if (e != hipSuccess && !why.empty()) return fail(c, STC_E_HIP, why);
   if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
   float ms_build = 0.f;
@@ -700,22 +722,43 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
   int8_t *ds = c->bb.in_s.as<int8_t>();
   uint64_t *d_ebase = c->batch_vbase.as<uint64_t>();
   uint32_t *d_vbase = (uint32_t *)(d_ebase + (k + 1));
-  uint32_t *d_bad = d_vbase + (k + 1);
+  BuildInput in;
+  in.u = du;
+  in.v = dv;
+  in.s = ds;
+  in.m_in = m;
+  in.n = n;
+  in.ebase = d_ebase;
+  in.vbase = d_vbase;
+  in.k = k;
+  std::string why;
   float ms_up = 0.f;
   const bool staged = use_staging(c, m);
-  e = hipStreamSynchronize(c->st);
+  // the graph ranges first: the per-chunk canonicalisation reads them
+  e = hipMemcpyAsync(d_ebase, ebase.data(), (k + 1) * 8, hipMemcpyHostToDevice, c->st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_vbase, vbase.data(), (k + 1) * 4, hipMemcpyHostToDevice, c->st);
+  if (e == hipSuccess) e = build_begin(in, c->bb, why, c->st);
+  if (e == hipErrorInvalidValue && !why.empty()) return fail(c, STC_E_ARG, why);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e != hipSuccess) return hip_fail(c, e, "batch upload");
   const auto t0 = std::chrono::steady_clock::now();
   // find the first graph with lines for the contiguous fast path
   uint32_t first = 0;
   while (first < k && graphs[first].m == 0) first++;
-  if (e == hipSuccess && staged) {
-    // the graphs back to back through the pinned staging buffers
+  bool canon_done = false;  // lines canonicalised chunk by chunk during the upload
+  if (staged) {
+    // the graphs back to back through the pinned staging buffers, each chunk
+    // shifted into the union's ids and canonicalised while the next copies
     std::vector<EdgePiece> pieces;
     pieces.reserve(k);
     for (uint32_t i = 0; i < k; i++)
       if (graphs[i].m) pieces.push_back(EdgePiece{graphs[i].u, graphs[i].v, graphs[i].sign, graphs[i].m});
-    e = c->ring.upload(pieces, du, dv, ds, c->st);
-  } else if (e == hipSuccess && m && contiguous && first < k) {
+    e = c->ring.upload(pieces, du, dv, ds, c->up, c->st, [&](uint64_t l0, uint64_t l1) {
+      return build_canon(in, l0, l1, c->bb, c->st);
+    });
+    canon_done = true;
+  } else if (m && contiguous && first < k) {
     e = hipMemcpyAsync(du, graphs[first].u, m * 4, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(dv, graphs[first].v, m * 4, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(ds, graphs[first].sign, m, hipMemcpyHostToDevice, c->st);
@@ -728,24 +771,15 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
       if (e == hipSuccess) e = hipMemcpyAsync(ds + o, graphs[i].sign, mi, hipMemcpyHostToDevice, c->st);
     }
   }
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_ebase, ebase.data(), (k + 1) * 8, hipMemcpyHostToDevice, c->st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_vbase, vbase.data(), (k + 1) * 4, hipMemcpyHostToDevice, c->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess && !staged) e = hipStreamSynchronize(c->st);
   ms_up = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (e != hipSuccess) return hip_fail(c, e, "batch upload");
-  // shift ids, build the union CSR
+  // the union CSR (ids shifted per graph by the canonicalisation)
   e = hipEventRecord(c->ev[0], c->st);
-  if (e == hipSuccess) e = shift_batch_ids(du, dv, m, d_ebase, d_vbase, k, d_bad, c->st);
-  uint32_t bad = 0;
-  if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, c->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-  if (e != hipSuccess) return hip_fail(c, e, "batch shift");
-  if (bad) return fail(c, STC_E_ARG, "a batch graph has an endpoint >= its n");
-  std::string why;
-  e = build_csr(du, dv, ds, m, n, d_vbase, k, c->bb, c->br, why, c->st);
+  if (e == hipSuccess && !canon_done) e = build_canon(in, 0, m, c->bb, c->st);
+  if (e == hipSuccess) e = build_finish(in, c->bb, c->br, why, c->st);
   if (e == hipErrorInvalidValue && !why.empty()) return fail(c, STC_E_ARG, why);
+  if (e != hipSuccess && !why.empty()) return fail(c, STC_E_HIP, why);
   if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
   if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
   float ms_build = 0.f;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -134,10 +135,13 @@ struct UploadRing {
   ~UploadRing() { release(); }
   hipError_t ensure();
   void release();
-  // Copy the pieces back to back into du / dv / ds (device) on stream st and
-  // wait for the copies to finish.
+  // Copy the pieces back to back into du / dv / ds (device) on stream cs and
+  // wait for the copies to finish.  After each chunk's copies, stream ks waits
+  // for them and on_chunk(l0, l1) (if set) enqueues work on ks for lines
+  // [l0, l1), which then overlaps the copies of the next chunks.
+  using ChunkFn = std::function<hipError_t(uint64_t, uint64_t)>;
   hipError_t upload(const std::vector<EdgePiece> &pieces, uint32_t *du, uint32_t *dv, int8_t *ds,
-                    hipStream_t st);
+                    hipStream_t cs, hipStream_t ks = nullptr, const ChunkFn &on_chunk = nullptr);
 };
 
 // ---- device memory --------------------------------------------------------
@@ -192,42 +196,53 @@ hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
                               ScanScratch &scr, hipStream_t s);
 // Bytes of ScanScratch level l that a scan of n words needs (0: level unused).
 size_t scan_level_bytes(uint64_t n, int l);
-// Stable LSD radix sort of 64-bit keys on the given 8-bit digit shifts
-// (ascending significance).  Result lands in *keys or *tmp; on return *keys
-// points at the sorted data.
-hipError_t radix_sort_u64(uint64_t **keys, uint64_t **tmp, uint64_t n,
-                          const std::vector<int> &shifts, DevBuf &hist,
-                          ScanScratch &scr, hipStream_t s);
+// Stable LSD radix sort of n < 2^32 64-bit keys on bits [lo_bit, lo_bit +
+// nbits), 8-bit digits; scratch holds the per-tile (4096 keys) digit counts.
+// On return *keys points at the sorted data (*tmp at the other buffer).
+hipError_t radix_sort_u64(uint64_t **keys, uint64_t **tmp, uint64_t n, int lo_bit, int nbits,
+                          DevBuf &scratch, hipStream_t s, bool first_counted);
+// Bytes of radix sort scratch for n keys; with first_counted the tile counts
+// of the first digit are already in scratch (written by the keys' producer).
+size_t radix_scratch_bytes(uint64_t n);
 
 struct BuildBuffers {
   DevBuf in_u, in_v, in_s;
   DevBuf keyA, keyB, flag, pos;
   DevBuf deg, vkeyA, vkeyB, rank, perm, graph_of;
   DevBuf col, src, off, fdeg, tin, tin_off, dcls, smask;
-  DevBuf hist, misc;
+  DevBuf misc, sort;
   ScanScratch scr;
 };
 
-// Builds the rank-space oriented CSR from a device edge list.  vbase (k+1
-// entries, device) gives per-graph vertex ranges for a disjoint union of k
-// graphs (nullptr for one graph).
+// A device edge list to build from: m_in lines (u, v, sign) over ids < n, or
+// for a disjoint union of k graphs (k >= 1) ebase / vbase (device, k + 1
+// entries each): graph g owns lines [ebase[g], ebase[g+1]) with ids below
+// vbase[g+1] - vbase[g], shifted by vbase[g] into the union's id space.
+struct BuildInput {
+  const uint32_t *u = nullptr, *v = nullptr;
+  const int8_t *s = nullptr;
+  uint64_t m_in = 0;
+  uint32_t n = 0;
+  const uint64_t *ebase = nullptr;
+  const uint32_t *vbase = nullptr;
+  uint32_t k = 0;
+};
 struct BuildResult {
   uint32_t n = 0;
   uint64_t m_in = 0, m = 0;
   uint32_t max_deg = 0, max_out = 0;
   double b_alg = 0;
 };
-hipError_t build_csr(const uint32_t *u, const uint32_t *v, const int8_t *s,
-                     uint64_t m_in, uint32_t n, const uint32_t *vbase_dev,
-                     uint32_t k, BuildBuffers &bb, BuildResult &res,
-                     std::string &err, hipStream_t st);
-
-// Shift the ids of k concatenated edge lists into the union id space (graph g
-// owns lines [ebase[g], ebase[g+1]) and ids [vbase[g], vbase[g+1])); sets
-// *bad_dev nonzero if a line has an id outside its graph.
-hipError_t shift_batch_ids(uint32_t *u, uint32_t *v, uint64_t m, const uint64_t *ebase_dev,
-                           const uint32_t *vbase_dev, uint32_t k, uint32_t *bad_dev,
-                           hipStream_t st);
+// Builds the rank-space oriented CSR (and the transposed index) on stream st
+// in three steps: build_begin sizes the buffers, build_canon turns lines
+// [l0, l1) into canonical sort keys and counts their first digit (l0 a
+// multiple of 4096; called once for all lines, or once per uploaded chunk as
+// soon as its copies are done), build_finish does the rest.
+hipError_t build_begin(const BuildInput &in, BuildBuffers &bb, std::string &err, hipStream_t st);
+hipError_t build_canon(const BuildInput &in, uint64_t l0, uint64_t l1, BuildBuffers &bb,
+                       hipStream_t st);
+hipError_t build_finish(const BuildInput &in, BuildBuffers &bb, BuildResult &res,
+                        std::string &err, hipStream_t st);
 
 struct CountBuffers {
   DevBuf gcount, lists, blk, vflag, vlist, bk8, pv, pv_ids;

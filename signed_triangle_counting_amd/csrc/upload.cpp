@@ -2,11 +2,12 @@
 //
 // The caller's arrays are ordinary (pageable) host memory.  They are copied
 // chunk by chunk into one of two pinned buffers by a few host threads, and
-// each filled buffer is sent to the device with asynchronous copies on the
-// counter's stream; while the DMA engine drains one buffer the threads fill
-// the other.  Pieces (one per graph of a batch) are packed back to back, so a
-// batch of thousands of small graphs costs a few large copies instead of three
-// small ones per graph.
+// each filled buffer is sent to the device with asynchronous copies on a copy
+// stream; while the DMA engine drains one buffer the threads fill the other.
+// A consumer of each chunk (the CSR build's canonicalisation) can run on the
+// compute stream as soon as that chunk has arrived.  Pieces (one per graph of
+// a batch) are packed back to back, so a batch of thousands of small graphs
+// costs a few large copies instead of three small ones per graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,7 +69,7 @@ void UploadRing::release() {
 }
 
 hipError_t UploadRing::upload(const std::vector<EdgePiece> &pc, uint32_t *du, uint32_t *dv,
-                              int8_t *ds, hipStream_t st) {
+                              int8_t *ds, hipStream_t st, hipStream_t ks, const ChunkFn &on_chunk) {
   std::vector<uint64_t> pre(pc.size() + 1, 0);
   for (size_t i = 0; i < pc.size(); i++) pre[i + 1] = pre[i] + pc[i].m;
   const uint64_t total = pre.back();
@@ -133,6 +134,11 @@ hipError_t UploadRing::upload(const std::vector<EdgePiece> &pc, uint32_t *du, ui
     if (e == hipSuccess)
       e = hipMemcpyAsync(ds + c0, h + kStageLines * 8, len, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(done[i & 1], st);
+    if (e == hipSuccess && on_chunk) {
+      // the chunk's consumer runs on ks once its copies are done
+      if (ks != st) e = hipStreamWaitEvent(ks, done[i & 1], 0);
+      if (e == hipSuccess) e = on_chunk(c0, c0 + len);
+    }
   }
   {
     std::lock_guard<std::mutex> lk(mu);

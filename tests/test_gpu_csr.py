@@ -80,3 +80,8 @@ def test_device_csr_equals_oracle_forward_csr(ctx, case):
     want_smask = np.zeros(n, np.int64)
     np.bitwise_or.at(want_smask, dst_e, bits)
     assert (a["smask"].astype(np.int64) == want_smask).all()
+    # the merge model's bytes, 4 * sum over oriented edges of (d+(u) + d+(v))
+    # + 4 per edge + 8 per vertex, and the largest out-degree
+    words = int((fdr[src_r] + fdr[dst_r]).sum())
+    assert st["b_alg"] == 4.0 * words + 4.0 * m + 8.0 * (n + 1)
+    assert st["max_out_degree"] == (int(fdr.max()) if n else 0)
