@@ -45,12 +45,11 @@ constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kerne
 // Device control block, cleared at the start of every counting run.
 struct Ctrl {
   uint32_t bucket_cnt[kBuckets];
-  uint32_t done;
   uint32_t len[8];   // merge-lane list, merge-wave list, then vertex lists:
                      // workgroup hash, workgroup bitmap, wave hash, wave bitmap,
                      // workgroup pull, wave pull
   uint32_t work[8];  // dynamic work counters
-  uint32_t pad[1];
+  uint32_t pad[2];
   uint32_t queue[7][8][32];          // XCD work queues (workgroup hash,
                                      // workgroup bitmap, wave hash, batch
                                      // phase 1, wave bitmap or batch phase
@@ -330,14 +329,10 @@ __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uin
 // ---- binning ----------------------------------------------------------------
 // Bucket histogram; every edge's bucket byte goes to bk8 (kNoBucket when
 // the edge cannot close a triangle) so the scatter pass need not recompute it.
-// (8 waves per SIMD, the launch's occupancy: the last workgroup's inference
-// would otherwise lift the register count past 64 and hold one workgroup per
-// CU back until another finished.)
 __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl,
                                                   const DispatchParams *P,
                                                   uint8_t *__restrict__ bk8) {
   __shared__ uint32_t h[kBuckets];
-  __shared__ int last;
   const int tid = threadIdx.x;
   h[tid] = 0;
   __syncthreads();
@@ -364,15 +359,16 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl,
   }
   __syncthreads();
   if (h[tid]) atomicAdd(&ctrl->bucket_cnt[tid], h[tid]);
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) last = atomicAdd(&ctrl->done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    uint32_t cnt = atomicAdd(&ctrl->bucket_cnt[tid], 0u);
-    ctrl->table[tid] = cnt ? decide(tid, cnt, *P) : (uint8_t)S_SKIP;
-  }
+}
+
+// The strategy table from the bucket histogram (one workgroup, thread =
+// bucket), after every workgroup of k_bin_hist has added its counts: a kernel
+// of its own, because the last-workgroup-decides pattern cost every workgroup
+// a device-scope fence.
+__global__ __launch_bounds__(kBuckets) void k_bin_table(Ctrl *ctrl, const DispatchParams *P) {
+  const int tid = threadIdx.x;
+  const uint32_t cnt = ctrl->bucket_cnt[tid];
+  ctrl->table[tid] = cnt ? decide(tid, cnt, *P) : (uint8_t)S_SKIP;
 }
 
 // Edge lists are written per workgroup: workgroup b owns the segment
@@ -2401,6 +2397,7 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   STC_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_bin_hist, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp,
                      cb.bk8.as<uint8_t>());
+  hipLaunchKernelGGL(k_bin_table, dim3(1), dim3(kBuckets), 0, st, ctrl, dp);
   STC_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_bin_scatter, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp,
                      cb.bk8.as<uint8_t>(), lane_seg, wave_seg, seg, blk_cnt, vf);
