@@ -95,8 +95,11 @@ typedef struct stc_stats {
   uint32_t max_out_degree;
   double b_alg;            /* sum over oriented edges of 4(d+(u)+d+(v)),
                               + 4 m+ + 8 (n+1) bytes */
-  float ms_upload;         /* host -> device copy of the edge list */
-  float ms_build;          /* on-device CSR build + orientation */
+  float ms_upload;         /* host -> device copy of the edge list (staged copies
+                              also canonicalise each chunk on the device as it
+                              arrives) */
+  float ms_build;          /* on-device CSR build + orientation, from the end of
+                              the upload */
   float ms_count;          /* binning + intersection, median over repeats */
   float ms_count_min;
   float ms_bin;            /* binning part of the last run */
