@@ -1146,16 +1146,20 @@ __device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, u
 // 2 span, and kEmpty - 2 lo exceeds 2 span too, because the API admits at most
 // 2^31 - 2^20 vertices (ranks).  LOW = false is the probe of a list with no
 // entries below the window (nlow = 0), which skips the low test entirely.
+// The word is read whatever the window test says, at an index clamped to the
+// window's last word (lastw), and zeroed after: with no branch around the
+// read, the reads of an unrolled group go out together instead of each
+// waiting for the one before.
 template <bool LOW>
 struct BitmapProbe {
   const uint32_t *B;
-  uint32_t lo2, span2;
+  uint32_t lo2, span2, lastw;
   const uint32_t *L;
   uint32_t nlow;
   __device__ __forceinline__ void operator()(uint32_t y, bool &same, bool &diff) const {
     const uint32_t d2 = y - lo2;
-    uint32_t w = 0;
-    if (d2 < span2) w = B[d2 >> 5];
+    uint32_t w = B[min(d2 >> 5, lastw)];
+    w = d2 < span2 ? w : 0u;
     // v_bfe_u32 reads bits [4:0] of its offset operand: d2 & 31 needs no mask
     same = __builtin_amdgcn_ubfe(w, d2, 1u) != 0u;
     diff = __builtin_amdgcn_ubfe(w, d2 ^ 1u, 1u) != 0u;
@@ -1644,9 +1648,9 @@ struct Staged {
                                              const F &f) const {
     const uint32_t lo2 = base << 1, span2 = (hi - base + 1u) << 1;
     if (nlow)
-      f(BitmapProbe<true>{B, lo2, span2, L, nlow});
+      f(BitmapProbe<true>{B, lo2, span2, (span2 - 1u) >> 5, L, nlow});
     else
-      f(BitmapProbe<false>{B, lo2, span2, L, 0u});
+      f(BitmapProbe<false>{B, lo2, span2, (span2 - 1u) >> 5, L, 0u});
   }
 };
 
