@@ -544,13 +544,18 @@ __device__ __forceinline__ void vertex_lists_of(uint32_t v, const DevCsr &g, con
   c[5] = pull && small ? (indeg + cw - 1) / cw : 0u;
 }
 
+// Inclusive sum over the wave (every lane active) by DPP: row_shr 1, 2, 4, 8
+// scan each row of 16 lanes, row_bcast 15 adds row 0's total to row 1 and
+// row 2's to row 3, row_bcast 31 adds the first two rows' total to rows 2
+// and 3.  No LDS round trip sits between the steps (a lane-shuffle scan takes
+// six dependent ds_bpermute reads).
 __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
-  const uint32_t lane = __lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
-    if (lane >= (uint32_t)d) x += y;
-  }
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return x;
 }
 
@@ -1435,15 +1440,20 @@ __device__ __forceinline__ uint32_t wave_max_scan_incl(uint32_t x) {
 // units of kPackUnit consecutive entries, which may span many lists, so a
 // unit's loads are full whatever the list lengths.  Entry f of the packed
 // sequence belongs to the last list i with P_i <= f and lies at col[f + pb_i
-// - P_i].  Per unit, each list starting inside the unit writes i + 1 at its
-// start slot of the wave's heads array H (kPackUnit bytes, one per slot);
-// each lane then carries the last head forward over its 8 consecutive slots
-// and a wave max-scan carries it across lanes, which leaves every slot's
-// list index in H.  Each entry's triangle class follows from the sign bits
-// of its three edges, so no per-list reduction is needed (callers that
-// credit triangles per edge owner use stream_edges).
+// - P_i].  The lists are numbered in order (their compact index c, from
+// mbcnt over the ballot of non-empty lists), and each list's record
+// {pb_i - P_i, its edge's sign} goes to H[c].  Per unit, every list starting
+// inside it sets bit f - B of a kPackUnit-bit start mask M (16 LDS words),
+// so slot f of group u (64 slots) lies in the list numbered S_u + (number of
+// start bits at or below f within the group) - 1, S_u being the lists
+// started before the group: one mbcnt over the group's 64-bit mask word,
+// with the counts of earlier groups carried in a scalar.  Each entry's
+// triangle class follows from the sign bits of its three edges, so no
+// per-list reduction is needed (callers that credit triangles per edge owner
+// use stream_edges).  H holds kPackWords words: 64 records, then M.
 constexpr uint32_t kPackUnit = 512;  // 64 lanes x 8 slots
 static_assert(kPackUnit == 64u * kUnroll, "packed unit: 8 slots per lane");
+constexpr uint32_t kPackWords = 2u * 64u + kPackUnit / 32u;
 
 template <bool CLS, class Meta, class Keep, class Probe>
 __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, uint32_t ea,
@@ -1451,7 +1461,7 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
                                               const Probe &probe, uint64_t *H, Acc<CLS> &acc) {
   const uint32_t *__restrict__ col = g.col;
   const uint32_t lane = __lane_id();
-  uint8_t *Hb = (uint8_t *)H;
+  uint32_t *const M = (uint32_t *)(H + 64);
   for (uint32_t j0 = first; j0 < ea; j0 += 64u * step) {
     const uint32_t j = j0 + lane * step;
     uint32_t w = 0, pb = 0, eb = 0, x = 0;
@@ -1462,45 +1472,46 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
     const uint32_t P = incl - len;
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     if (total == 0u) continue;
-    const uint32_t D = pb - P;  // modulo 2^32: f + D = pb + (f - P)
-    const uint64_t negm = __ballot(kept && (w & 1u));
-    // list index of each slot of the unit at B, and its loads (rounds past
-    // the end load nothing)
-    auto prepare = [&](uint32_t B, uint32_t(&y)[kUnroll], uint32_t(&li)[kUnroll]) {
-      wave_lds_sync();  // the previous unit's reads of H are done
-      H[lane] = 0ull;
+    const bool live = len != 0u;
+    const uint64_t lm = __ballot(live);
+    wave_lds_sync();  // the previous batch's reads of H and M are done
+    if (live) {
+      const uint32_t c = __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u));
+      H[c] = ((uint64_t)(w & 1u) << 32) | (uint32_t)(pb - P);  // f + (pb - P) mod 2^32
+    }
+    // the unit at B: start bits, then each slot's list record and its loads
+    // (groups past the end load nothing); n[u] is the list's edge sign
+    auto prepare = [&](uint32_t B, uint32_t(&y)[kUnroll], uint32_t(&n)[kUnroll]) {
+      if (B) wave_lds_sync();  // the previous unit's reads of M are done
+      if (lane < kPackUnit / 32u) M[lane] = 0u;
       wave_lds_sync();
-      if (len != 0u && P >= B && P - B < kPackUnit) Hb[P - B] = (uint8_t)(lane + 1u);
+      const uint32_t s = P - B;  // wraps for lists that started before B
+      if (live && s < kPackUnit) atomicOr(&M[s >> 5], 1u << (s & 31u));
       wave_lds_sync();
-      const uint64_t h = H[lane];
-      // the list covering slot B (it may start in an earlier unit)
-      const uint64_t cov = __ballot(len != 0u && P <= B);
-      const uint32_t run_in = cov ? 64u - (uint32_t)__clzll(cov) : 0u;
-      uint32_t last = 0;
-#pragma unroll
-      for (int k = 0; k < 8; k++) last = max(last, (uint32_t)(h >> (8 * k)) & 0xFFu);
-      const uint32_t scan = wave_max_scan_incl(last);
-      const uint32_t before = (uint32_t)__shfl_up((int)scan, 1, 64);
-      uint32_t run = max(run_in, lane ? before : 0u);
-      uint64_t r = 0;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        run = max(run, (uint32_t)(h >> (8 * k)) & 0xFFu);
-        r |= (uint64_t)run << (8 * k);
-      }
-      wave_lds_sync();
-      H[lane] = r;
-      wave_lds_sync();
+      const uint32_t mw = lane < kPackUnit / 32u ? M[lane] : 0u;
+      // lists started before the unit, minus one (wraps to ~0 at B = 0,
+      // where slot 0 always starts a list)
+      uint32_t S = (uint32_t)__popcll(__ballot(live && P < B)) - 1u;
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (B + u * 64u >= total) break;  // wave-uniform
+        const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mw, 2 * u + 1) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)mw, 2 * u);
+        // start bits at or below this lane: bit 0, plus the bits of m >> 1 below it
+        const uint64_t m1 = m >> 1;
+        const uint32_t c = S + (uint32_t)(m & 1u) +
+                           __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+        S += (uint32_t)__popcll(m);
+        // c < 64 on every lane (at most 64 lists start at or below any slot)
+        const uint64_t rec = H[c & 63u];
         const uint32_t f = B + u * 64u + lane;
-        li[u] = (uint32_t)Hb[u * 64u + lane] - 1u;
-        const uint32_t d = (uint32_t)__shfl((int)D, (int)(li[u] & 63u), 64);
-        y[u] = f < total ? col[f + d] : kEmpty;
+        y[u] = f < total ? col[f + (uint32_t)rec] : kEmpty;
+        n[u] = (uint32_t)(rec >> 32);
       }
     };
-    auto consume = [&](uint32_t B, const uint32_t(&y)[kUnroll], const uint32_t(&li)[kUnroll]) {
+    auto consume = [&](uint32_t B, const uint32_t(&y)[kUnroll], const uint32_t(&n)[kUnroll]) {
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (B + u * 64u >= total) break;  // wave-uniform
@@ -1508,18 +1519,17 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
         probe(y[u], same, diff);
         // negatives among the triangle's three edges: the list's edge, plus
         // 1 (diff) or 2 nb (same) for the two edges to the found vertex
-        const uint32_t neg = (uint32_t)((negm >> (li[u] & 63u)) & 1ull) +
-                             (diff ? 1u : 2u * (y[u] & 1u));
+        const uint32_t neg = n[u] + (diff ? 1u : 2u * (y[u] & 1u));
         acc.add(same || diff, neg);
       }
     };
     // One unit at a time: a unit already holds up to kPackUnit entries of many
     // lists, and a second buffer in flight measured no faster while it cost
     // the registers that let the batched kernel run packed pull items.
-    uint32_t y[kUnroll], li[kUnroll];
+    uint32_t y[kUnroll], n[kUnroll];
     for (uint32_t B = 0; B < total; B += kPackUnit) {
-      prepare(B, y, li);
-      consume(B, y, li);
+      prepare(B, y, n);
+      consume(B, y, n);
     }
   }
 }
@@ -1811,7 +1821,7 @@ __device__ __forceinline__ PullChunk pull_chunk(const DevCsr &g, uint32_t j0, ui
 }
 
 // Pull streaming.  PACKED: the lists are streamed packed (H: the wave's
-// kPackUnit-byte heads array), which pays off for the short lists of the
+// kPackWords-word records and start masks), which pays off for the short lists of the
 // wave kernel's targets; the workgroup kernel's targets stream lists long
 // enough for the list-at-a-time path, which spends fewer instructions per
 // entry.  Per-vertex counts credit each edge's owner and always take the
@@ -1847,16 +1857,67 @@ __device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, 
   unstage_block(g, st, B);
 }
 
-// One wave: the in-edges [j0, t1) of a target whose list is col[la, ea).
+// One wave: the in-edges [j0, t1) of a target whose list is col[la, ea) (at
+// most kWaveMaxDeg entries).  The list's entries (one per lane, whose first
+// and last give the rank window), the first 64 in-edge records and the
+// target class's pull mask are fetched together, so an item starts with one
+// memory round trip instead of a chain of four.  The mask has bit c set when
+// bucket (c, class of d+(v)) is binned to pull: an in-edge is pull iff its
+// suffix length's class c has its bit set.
 template <bool CLS>
 __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, uint32_t t1,
                                                  uint32_t la, uint32_t ea, uint32_t *B,
                                                  uint32_t *L, uint32_t window, uint64_t *H,
                                                  const uint8_t *table, Acc<CLS> &acc) {
-  const StagedWave sw = stage_wave_at(g, la, ea, B, L, window);
-  const Staged &st = sw.st;
+  if (CLS && g.pv) {  // per-vertex counts credit each edge's owner (stream_edges)
+    const StagedWave sw = stage_wave_at(g, la, ea, B, L, window);
+    const Staged &st = sw.st;
+    st.with_probe(B, L, [&](const auto &probe) {
+      pull_stream<CLS, true>(g, j0, t1, 1u, st.d, table, probe, H, acc);
+    });
+    unstage_wave(sw, B);
+    return;
+  }
+  const uint32_t *__restrict__ col = g.col;
+  const uint32_t lane = __lane_id();
+  const uint32_t d = ea - la;
+  uint32_t w = 0;
+  if (lane < d) w = col[la + lane];
+  InEdge r0{0u, 0u};
+  if (j0 + lane < t1) r0 = g.tin[j0 + lane];
+  const int cb = min(clog2(d), kBucketSide - 1);
+  const uint32_t pmask =
+      (uint32_t)__ballot(lane < (uint32_t)kBucketSide && table[lane * kBucketSide + cb] == S_PULL);
+  // the window from the list's first and last entries
+  StagedWave sw;
+  Staged &st = sw.st;
+  st.la = la;
+  st.ea = ea;
+  st.d = d;
+  st.lo = (uint32_t)__builtin_amdgcn_readlane((int)w, 0) >> 1;
+  st.hi = (uint32_t)__builtin_amdgcn_readlane((int)w, (int)d - 1) >> 1;
+  st.span = st.hi - st.lo + 1u;
+  st.base = window_base(st.lo, st.hi, window);
+  const bool mine = lane < d;
+  sw.o = (w >> 1) - st.base;  // wraps for entries below the window
+  sw.in_w = mine && (w >> 1) >= st.base;
+  if (sw.in_w) atomicOr(&B[sw.o >> 4], 1u << ((sw.o & 15u) * 2u + (w & 1u)));
+  else if (mine) L[lane] = w;  // the low entries are the lanes [0, nlow)
+  st.nlow = (uint32_t)__popcll(__ballot(mine && !sw.in_w));
+  wave_lds_sync();
+  // the first metadata batch's records are the ones fetched above
+  const auto meta = [&](uint32_t j, uint32_t &mw, uint32_t &pb, uint32_t &eb, uint32_t &x) {
+    const InEdge r = j - j0 < 64u ? r0 : g.tin[j];
+    mw = r.len_neg & 1u;
+    pb = r.pos;
+    eb = r.pos + (r.len_neg >> 1);
+    x = 0u;
+  };
+  const auto keep = [&](uint32_t, uint32_t pb, uint32_t eb) {
+    return eb > pb && ((pmask >> min(clog2(eb - pb), kBucketSide - 1)) & 1u);
+  };
   st.with_probe(B, L, [&](const auto &probe) {
-    pull_stream<CLS, true>(g, j0, t1, 1u, st.d, table, probe, H, acc);
+    stream_packed<CLS>(g, j0, t1, 1u, meta, keep, probe, H, acc);
   });
   unstage_wave(sw, B);
 }
@@ -2008,7 +2069,7 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   extern __shared__ uint32_t lds[];
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
-  __shared__ uint64_t s_heads[PULL ? 256 : 1];  // packed streaming, 64 per wave
+  __shared__ uint64_t s_heads[PULL ? 4 * kPackWords / 2 : 1];  // packed streaming, per wave
   const uint32_t len = ctrl->len[PULL ? 7 : 5];
   if (len == 0) return;  // the whole grid: no vertex has this kernel's edges
   // defer (pull): the two-phase launch that follows counts this list with the
@@ -2041,7 +2102,7 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
         const uint32_t j0 = head_of(heads, 0, it);
         pull_vertex_wave<CLS>(g, j0, min(j0 + (1u << tune.pull_wave_chunk_log2), head_of(heads, 4, it)),
                               head_of(heads, 2, it), head_of(heads, 3, it), B, L, window,
-                              s_heads + wave * 64u, table, acc);
+                              s_heads + wave * (kPackWords / 2u), table, acc);
       } else {
         bitmap_vertex_wave<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first + it]), B, L,
                                 window, table, tune, acc);
@@ -2086,10 +2147,11 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 //   All LDS structures share one region R, kept zero between items.  Phase 1
 // uses the workgroup bitmap R[0, bitmap_words) with its low array after it,
 // or the workgroup hash table R[0, hash_slots).  In phase 2 wave w owns the
-// slot R[w S, (w + 1) S), S = window_words + kWaveMaxDeg + kPackUnit / 4:
+// slot R[w S, (w + 1) S), S = window_words + kWaveMaxDeg + kPackWords:
 // its hash table or window bitmap at the start, the window's low array after
-// the window, and its packed-streaming heads array at the end (the heads need
-// no clearing: phase 1, which needs zeros there, is over by then).  Bitmaps
+// the window, and its packed-streaming records and start masks at the end
+// (they need no clearing: phase 1, which needs zeros there, is over by then,
+// and the streaming clears the masks itself).  Bitmaps
 // clear what they set; tables and low arrays are cleared after use.
 constexpr uint32_t kBatchBlock = 512;
 constexpr uint32_t kWavesPerBatchBlock = kBatchBlock / 64;
@@ -2197,7 +2259,7 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
   }
   // phase 2: wave items [merge-wave | wave hash | wave bitmap | wave pull]
   {
-    const uint32_t slot = window_words + kWaveMaxDeg + kPackUnit / 4u;
+    const uint32_t slot = window_words + kWaveMaxDeg + kPackWords;
     uint32_t *const W = R + wave * slot;
     uint32_t *const LW = W + window_words;
     uint64_t *const HW = (uint64_t *)(LW + kWaveMaxDeg);
@@ -2578,7 +2640,7 @@ static size_t batch_region_words(const TuneParams &t) {
   const size_t hash = (size_t)4u << t.hash_max_log2;
   // phase 2: one slot per wave (hash table or window + low array, then heads)
   const size_t wave_slot = std::max<size_t>(kWaveCap, (1u << t.wave_window_log2) / 16u) +
-                           kWaveMaxDeg + kPackUnit / 4u;
+                           kWaveMaxDeg + kPackWords;
   const size_t waves = (size_t)kWavesPerBatchBlock * wave_slot;
   return std::max(bitmap, std::max(hash, waves));
 }
