@@ -1135,9 +1135,11 @@ __device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, u
 // ---- probes of a staged list ----------------------------------------------------
 // probe(y, same, diff): whether the streamed entry y = (rank << 1) | nb is in
 // the staged list with the same sign (same) or the opposite sign (diff).
-// y = 0 and y = kEmpty always miss.  0 is no streamed entry (every streamed
-// rank lies above the vertex whose list holds it), and the streaming loops
-// leave the lanes of a buffer load past the list end, which read 0, unmasked.
+// y = 0 and y = kEmpty always miss, and so does kEmpty - 1 under the bitmap
+// probe (packed streaming flips the sign bit of its kEmpty padding).  0 is no
+// streamed entry (every streamed rank lies above the vertex whose list holds
+// it), and the streaming loops leave the lanes of a buffer load past the list
+// end, which read 0, unmasked.
 
 // LDS bitmap B over the rank window [lo, lo + span) of the staged list, two
 // bits per rank, one-hot by sign: rank lo + r with sign s sets bit 2r + s.
@@ -1148,7 +1150,7 @@ __device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, u
 // window fail it.  A 64-element group with a
 // target below lo (a prefix of the sorted N+(v), so rare) also searches L for
 // those lanes.  A target below lo wraps d2 to at least 2^32 - 2 lo > 2^21 >=
-// 2 span, and kEmpty - 2 lo exceeds 2 span too, because the API admits at most
+// 2 span, and kEmpty - 1 - 2 lo exceeds 2 span too, because the API admits at most
 // 2^31 - 2^20 vertices (ranks).  LOW = false is the probe of a list with no
 // entries below the window (nlow = 0), which skips the low test entirely.
 // The word is read whatever the window test says, at an index clamped to the
@@ -1512,15 +1514,30 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
       }
     };
     auto consume = [&](uint32_t B, const uint32_t(&y)[kUnroll], const uint32_t(&n)[kUnroll]) {
+      uint32_t even = 0, odd = 0;
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (B + u * 64u >= total) break;  // wave-uniform
         bool same, diff;
-        probe(y[u], same, diff);
-        // negatives among the triangle's three edges: the list's edge, plus
-        // 1 (diff) or 2 nb (same) for the two edges to the found vertex
-        const uint32_t neg = n[u] + (diff ? 1u : 2u * (y[u] & 1u));
-        acc.add(same || diff, neg);
+        if constexpr (CLS) {
+          probe(y[u], same, diff);
+          // negatives among the triangle's three edges: the list's edge, plus
+          // 1 (diff) or 2 nb (same) for the two edges to the found vertex
+          const uint32_t neg = n[u] + (diff ? 1u : 2u * (y[u] & 1u));
+          acc.add(same || diff, neg);
+        } else {
+          // two classes: the list edge's sign goes into the entry's sign bit
+          // (its rank is unchanged), so that "same" reports the found entries
+          // whose triangle has an even number of negative edges; kEmpty turns
+          // into kEmpty - 1, which misses as kEmpty does
+          probe(y[u] ^ n[u], same, diff);
+          even += (uint32_t)__popcll(__ballot(same));
+          odd += (uint32_t)__popcll(__ballot(diff));
+        }
+      }
+      if constexpr (!CLS) {
+        acc.c[0] += even;
+        acc.c[1] += odd;
       }
     };
     // One unit at a time: a unit already holds up to kPackUnit entries of many
