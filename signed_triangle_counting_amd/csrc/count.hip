@@ -559,6 +559,11 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
   return x;
 }
 
+// Sum over the wave (every lane active), wave-uniform.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_incl(x), 63);
+}
+
 // Upper bound of the entries of one list: every vertex once plus the extra
 // chunks of the pull lists (at most m / kPullChunkMin).
 __host__ __device__ inline uint64_t vlist_stride(uint32_t n, uint64_t m) {
@@ -907,24 +912,24 @@ __device__ __forceinline__ void pv_credit(const DevCsr &g, uint32_t a, const Acc
 // list with the same sign (same: the two edges to x carry 0 or 2 negatives,
 // 2 nb of them) or the opposite sign (diff: exactly 1 negative).
 template <bool CLS>
-__device__ __forceinline__ void tally(bool same, bool diff, uint32_t nb, uint32_t &e0,
+__device__ __forceinline__ void tally(uint32_t same, uint32_t diff, uint32_t nb, uint32_t &e0,
                                       uint32_t &e1, uint32_t &e2) {
   if (CLS) {
-    e0 += (uint32_t)__popcll(__ballot(same && nb == 0u));
-    e1 += (uint32_t)__popcll(__ballot(diff));
-    e2 += (uint32_t)__popcll(__ballot(same && nb != 0u));
+    e0 += (uint32_t)__popcll(__ballot(same != 0u && nb == 0u));
+    e1 += (uint32_t)__popcll(__ballot(diff != 0u));
+    e2 += (uint32_t)__popcll(__ballot(same != 0u && nb != 0u));
   } else {
-    e0 += (uint32_t)__popcll(__ballot(same));  // even
-    e1 += (uint32_t)__popcll(__ballot(diff));  // odd
+    e0 += (uint32_t)__popcll(__ballot(same != 0u));  // even
+    e1 += (uint32_t)__popcll(__ballot(diff != 0u));  // odd
   }
 }
 
-// same / diff of a found staged word x ((rank << 1) | neg) against a
-// streamed entry with sign nb.
-__device__ __forceinline__ void found_as(bool found, uint32_t x, uint32_t nb, bool &same,
-                                         bool &diff) {
-  same = found && ((x ^ nb) & 1u) == 0u;
-  diff = found && ((x ^ nb) & 1u) != 0u;
+// same / diff (0 or 1) of a found staged word x ((rank << 1) | neg) against
+// a streamed entry with sign nb.
+__device__ __forceinline__ void found_as(bool found, uint32_t x, uint32_t nb, uint32_t &same,
+                                         uint32_t &diff) {
+  same = found && ((x ^ nb) & 1u) == 0u ? 1u : 0u;
+  diff = found && ((x ^ nb) & 1u) != 0u ? 1u : 0u;
 }
 
 // ---- merge: one lane per edge ----------------------------------------------
@@ -1133,13 +1138,14 @@ __device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, u
 }
 
 // ---- probes of a staged list ----------------------------------------------------
-// probe(y, same, diff): whether the streamed entry y = (rank << 1) | nb is in
-// the staged list with the same sign (same) or the opposite sign (diff).
-// y = 0 and y = kEmpty always miss, and so does kEmpty - 1 under the bitmap
-// probe (packed streaming flips the sign bit of its kEmpty padding).  0 is no
-// streamed entry (every streamed rank lies above the vertex whose list holds
-// it), and the streaming loops leave the lanes of a buffer load past the list
-// end, which read 0, unmasked.
+// probe(y, s, same, diff): whether the streamed entry y = (rank << 1) | nb is
+// in the staged list with the sign nb ^ s (same = 1) or the other sign (diff
+// = 1).  With s = 0 that is the entry's own sign.  Two-class counting passes
+// the sign of the streamed list's own edge as s: a hit then has an even
+// number of negative edges in its triangle exactly when same = 1.  y = 0 and
+// y = kEmpty always miss.  0 is no streamed entry (every streamed rank lies
+// above the vertex whose list holds it), and the streaming loops leave the
+// lanes of a buffer load past the list end, which read 0, unmasked.
 
 // LDS bitmap B over the rank window [lo, lo + span) of the staged list, two
 // bits per rank, one-hot by sign: rank lo + r with sign s sets bit 2r + s.
@@ -1163,19 +1169,22 @@ struct BitmapProbe {
   uint32_t lo2, span2, lastw;
   const uint32_t *L;
   uint32_t nlow;
-  __device__ __forceinline__ void operator()(uint32_t y, bool &same, bool &diff) const {
+  __device__ __forceinline__ void operator()(uint32_t y, uint32_t s, uint32_t &same,
+                                             uint32_t &diff) const {
     const uint32_t d2 = y - lo2;
     uint32_t w = B[min(d2 >> 5, lastw)];
     w = d2 < span2 ? w : 0u;
-    // v_bfe_u32 reads bits [4:0] of its offset operand: d2 & 31 needs no mask
-    same = __builtin_amdgcn_ubfe(w, d2, 1u) != 0u;
-    diff = __builtin_amdgcn_ubfe(w, d2 ^ 1u, 1u) != 0u;
+    // v_bfe_u32 reads bits [4:0] of its offset operand: d2 & 31 needs no mask;
+    // s flips bit 0, the sign half of the rank's two bits
+    const uint32_t ds = d2 ^ s;
+    same = __builtin_amdgcn_ubfe(w, ds, 1u);
+    diff = __builtin_amdgcn_ubfe(w, ds ^ 1u, 1u);
     if constexpr (LOW) {
       const bool below = y - 1u < lo2 - 1u;  // 0 < y < lo2: 0 is no entry
       if (__ballot(below)) {                  // wave-uniform
         if (below) {
           const uint32_t x = low_lookup(L, nlow, y >> 1);
-          found_as(x != kEmpty, x, y, same, diff);
+          found_as(x != kEmpty, x, y ^ s, same, diff);
         }
       }
     }
@@ -1186,7 +1195,8 @@ struct BitmapProbe {
 struct HashProbe {
   const uint32_t *T;
   uint32_t mask;
-  __device__ __forceinline__ void operator()(uint32_t y, bool &same, bool &diff) const {
+  __device__ __forceinline__ void operator()(uint32_t y, uint32_t s, uint32_t &same,
+                                             uint32_t &diff) const {
     uint32_t hit = kEmpty;
     if (y != kEmpty && y != 0u) {
       const uint32_t yr = y >> 1;
@@ -1201,7 +1211,7 @@ struct HashProbe {
         h = (h + 1) & mask;
       }
     }
-    found_as(hit != kEmpty, hit, y, same, diff);
+    found_as(hit != kEmpty, hit, y ^ s, same, diff);
   }
 };
 
@@ -1253,8 +1263,8 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       if (c.pos + u * 64u >= c.eb) break;  // wave-uniform
-      bool same, diff;
-      probe(y[u], same, diff);  // lanes past the list end read 0, which misses
+      uint32_t same, diff;
+      probe(y[u], 0u, same, diff);  // lanes past the list end read 0, which misses
       tally<CLS>(same, diff, y[u] & 1u, e0, e1, e2);
     }
     // Retire the buffer's loads here (a no-op use), so a group skipped past
@@ -1330,7 +1340,14 @@ __device__ __forceinline__ void stream_edges(const DevCsr &g, uint32_t first, ui
 // The lanes locate 64 rows at a time: row r belongs to the first lane whose
 // inclusive row count exceeds r (a binary search over the scan with lane
 // shuffles); each row's triangles are added at once with its edge's sign, so
-// no per-list state crosses rows.
+// no per-list state crosses rows.  Two-class counting folds the row's edge
+// sign into the probe and keeps per-lane hit counts (even, odd), summed over
+// the wave once per call: the per-row work is then vector instructions only,
+// where ballot, popcount and a sign-selected 64-bit add per row kept the
+// scalar unit busier than the vector unit.  A lane's count in one call is at
+// most the rows it probes, at most 2^26 + 2^13 (pull streams one chunk of at
+// most 2^13 in-edges of one target, whose suffixes are disjoint parts of the
+// column array of fewer than 2^32 entries).
 constexpr uint32_t kRows = kUnroll;
 static_assert(64u % kRows == 0u, "a unit of rows never straddles two groups of 64 rows");
 
@@ -1342,6 +1359,7 @@ __device__ __forceinline__ void stream_rows(const DevCsr &g, uint32_t first, uin
   const uint32_t lane = __lane_id();
   auto lane_of = [](uint32_t v, int i) { return (uint32_t)__builtin_amdgcn_readlane((int)v, i); };
   auto shfl = [](uint32_t v, uint32_t i) { return (uint32_t)__shfl((int)v, (int)i, 64); };
+  uint32_t hit_even = 0, hit_odd = 0;  // two classes: this lane's hits
   for (uint32_t j0 = first; j0 < ea; j0 += 64u * step) {
     const uint32_t j = j0 + lane * step;
     uint32_t w = 0, pb = 0, eb = 0, x = 0;
@@ -1391,11 +1409,18 @@ __device__ __forceinline__ void stream_rows(const DevCsr &g, uint32_t first, uin
 #pragma unroll
       for (uint32_t u = 0; u < kRows; u++) {
         if (R + u >= total) break;  // wave-uniform
-        bool same, diff;
-        probe(y[u], same, diff);  // lanes past the row's end read 0, which misses
-        uint32_t e0 = 0, e1 = 0, e2 = 0;
-        tally<CLS>(same, diff, y[u] & 1u, e0, e1, e2);
-        acc.add_edge((sabm >> u) & 1u, e0, e1, e2);
+        uint32_t same, diff;
+        // lanes past the row's end read 0, which misses
+        if constexpr (CLS) {
+          probe(y[u], 0u, same, diff);
+          uint32_t e0 = 0, e1 = 0, e2 = 0;
+          tally<CLS>(same, diff, y[u] & 1u, e0, e1, e2);
+          acc.add_edge((sabm >> u) & 1u, e0, e1, e2);
+        } else {
+          probe(y[u], (sabm >> u) & 1u, same, diff);
+          hit_even += same;
+          hit_odd += diff;
+        }
       }
       // retire the unit's loads here, so none is left pending past its use
 #pragma unroll
@@ -1423,6 +1448,10 @@ __device__ __forceinline__ void stream_rows(const DevCsr &g, uint32_t first, uin
       }
       R += kRows;
     }
+  }
+  if constexpr (!CLS) {
+    acc.c[0] += wave_sum(hit_even);
+    acc.c[1] += wave_sum(hit_odd);
   }
 }
 
@@ -1518,21 +1547,20 @@ __device__ __forceinline__ void stream_packed(const DevCsr &g, uint32_t first, u
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         if (B + u * 64u >= total) break;  // wave-uniform
-        bool same, diff;
+        uint32_t same, diff;
         if constexpr (CLS) {
-          probe(y[u], same, diff);
+          probe(y[u], 0u, same, diff);
           // negatives among the triangle's three edges: the list's edge, plus
           // 1 (diff) or 2 nb (same) for the two edges to the found vertex
           const uint32_t neg = n[u] + (diff ? 1u : 2u * (y[u] & 1u));
-          acc.add(same || diff, neg);
+          acc.add((same | diff) != 0u, neg);
         } else {
-          // two classes: the list edge's sign goes into the entry's sign bit
-          // (its rank is unchanged), so that "same" reports the found entries
-          // whose triangle has an even number of negative edges; kEmpty turns
-          // into kEmpty - 1, which misses as kEmpty does
-          probe(y[u] ^ n[u], same, diff);
-          even += (uint32_t)__popcll(__ballot(same));
-          odd += (uint32_t)__popcll(__ballot(diff));
+          // two classes: the list edge's sign folded into the probe, so that
+          // "same" reports the hits whose triangle has an even number of
+          // negative edges
+          probe(y[u], n[u], same, diff);
+          even += (uint32_t)__popcll(__ballot(same != 0u));
+          odd += (uint32_t)__popcll(__ballot(diff != 0u));
         }
       }
       if constexpr (!CLS) {
