@@ -390,7 +390,11 @@ __device__ __forceinline__ uint32_t lds_append(bool pred, uint32_t *s_len) {
 // atomic, only when they are not set yet.  Pull edges need nothing here: the
 // vertex-list pass finds their targets from each target's suffix-class mask
 // (a per-edge flag on the target cost one device-scope atomic per pull edge:
-// 23 ms of a 92 ms batch of 605 M edges).
+// 23 ms of a 92 ms batch of 605 M edges).  An edge whose bucket is pull with
+// a target class that always fits (the common case) is settled from its
+// bucket byte alone, without the list offsets; and a table whose buckets are
+// all pull (of such classes) or skipped leaves nothing to scatter at all, so
+// the workgroups return at once.
 __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
                                                      const uint8_t *__restrict__ bk8,
@@ -404,12 +408,18 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
   __shared__ uint32_t s_len[2];
   const int tid = threadIdx.x;
   const uint32_t lane = __lane_id();
-  table[tid] = ctrl->table[tid];
+  const uint8_t mine = ctrl->table[tid];
+  table[tid] = mine;
   if (tid == 0) {
     tune = P->tune;
     s_len[0] = s_len[1] = 0;
   }
-  __syncthreads();
+  // a bucket that lists merge edges or flags sources, or may demote pull
+  const bool other = mine != S_SKIP && (mine != S_PULL || tid % kBucketSide > kPullMaxClass);
+  if (!__syncthreads_or(other)) {
+    if (tid == 0) blk_cnt[blockIdx.x] = blk_cnt[gridDim.x + blockIdx.x] = 0u;
+    return;
+  }
   uint32_t *lane_out = lane_seg + (uint64_t)blockIdx.x * seg;
   uint32_t *wave_out = wave_seg + (uint64_t)blockIdx.x * seg;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -418,27 +428,33 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
     uint8_t s = S_SKIP;
     uint32_t a = 0xFFFFFFFFu;
     if (e < g.m) {
-      a = g.src[e];
       const uint32_t bk = bk8[e];
       if (bk != kNoBucket) {
-        const uint32_t la = g.off[a], ea = g.off[a + 1];
-        s = bucket_strategy(
-            table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
-            [&] {
-              const uint32_t b = g.col[e] >> 1;
-              return span_of(g.col, g.off[b], g.off[b + 1]);
-            },
-            (int)bk);
+        s = table[bk];
+        if (s != S_PULL || (int)(bk % kBucketSide) > kPullMaxClass || !tune.allow_pull) {
+          a = g.src[e];
+          const uint32_t la = g.off[a], ea = g.off[a + 1];
+          s = bucket_strategy(
+              table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
+              [&] {
+                const uint32_t b = g.col[e] >> 1;
+                return span_of(g.col, g.off[b], g.off[b + 1]);
+              },
+              (int)bk);
+        }
       }
     }
     uint32_t p = lds_append(s == S_MERGE_LANE, &s_len[0]);
     if (s == S_MERGE_LANE) lane_out[p] = (uint32_t)e;
     p = lds_append(s == S_MERGE_WAVE, &s_len[1]);
     if (s == S_MERGE_WAVE) wave_out[p] = (uint32_t)e;
-    // runs of equal sources: heads, and each head's run as a lane mask
+    const uint64_t hb = __ballot(s == S_HASH), bb = __ballot(s == S_BITMAP);
+    if ((hb | bb) == 0ull) continue;  // wave-uniform: no source to flag
+    // runs of equal sources: heads, and each head's run as a lane mask (a
+    // is loaded for every edge of the wave here)
+    if (e < g.m) a = g.src[e];
     const uint32_t prev = (uint32_t)__shfl_up((int)a, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || prev != a);
-    const uint64_t hb = __ballot(s == S_HASH), bb = __ballot(s == S_BITMAP);
     if ((heads >> lane) & 1u) {
       const uint64_t above = heads & ~((2ull << lane) - 1ull);
       const uint64_t run = (above ? (1ull << __builtin_ctzll(above)) - 1ull : ~0ull) &
