@@ -2081,8 +2081,10 @@ __global__ __launch_bounds__(256) void k_hash_wave(DevCsr g, Ctrl *ctrl,
     if (lane == 0 &&
         !queue_pop<true>(ctrl->queue[2], len, (uint32_t)tune.wave_pop, cur, first, count))
       count = 0;
-    first = __shfl(first, 0, 64);
-    count = __shfl(count, 0, 64);
+    // lane 0 popped: readfirstlane makes the pop wave-uniform to the
+    // compiler, so the item loop runs on scalar registers
+    first = __builtin_amdgcn_readfirstlane(first);
+    count = __builtin_amdgcn_readfirstlane(count);
     if (count == 0) break;
     for (uint32_t it = 0; it < count; it++)
       hash_vertex_wave<CLS>(g, __builtin_amdgcn_readfirstlane(vlist[first + it]), T, table, tune,
@@ -2153,8 +2155,10 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
         !queue_pop<true>(ctrl->queue[PULL ? 6 : 4], len, (uint32_t)tune.wave_pop, cur, first,
                          count))
       count = 0;
-    first = __shfl(first, 0, 64);
-    count = __shfl(count, 0, 64);
+    // lane 0 popped: readfirstlane makes the pop wave-uniform to the
+    // compiler, so the item loop runs on scalar registers
+    first = __builtin_amdgcn_readfirstlane(first);
+    count = __builtin_amdgcn_readfirstlane(count);
     if (count == 0) break;
     const uint32_t heads =
         PULL ? pull_heads(g, vlist, vtarget, nullptr, first, 0u, count) : 0u;
@@ -2336,8 +2340,10 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
                                                          (uint32_t)tune.wave_pop, cur, first,
                                                          count))
         count = 0;
-      first = __shfl(first, 0, 64);
-      count = __shfl(count, 0, 64);
+      // lane 0 popped: readfirstlane makes the pop wave-uniform to the
+      // compiler, so the item loop runs on scalar registers
+      first = __builtin_amdgcn_readfirstlane(first);
+      count = __builtin_amdgcn_readfirstlane(count);
       if (count == 0) break;
       // headers of the pop's wave-pull items (items skip.. of the pop)
       const uint32_t skip = first >= w3 ? 0u : min(w3 - first, count);
