@@ -158,11 +158,10 @@ int stc_count_prepared(stc_ctx *ctx, const stc_opts *opts, stc_counts *out);
 /* stc_count_prepared without any device allocation, for callers that cannot
  * allocate device memory (code under HIP stream capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
- * counting pass's buffers (control block, bucket bytes, edge and vertex
- * lists, scan levels) come from one static device array of 128 MiB that the
- * library holds on every GPU it runs on until the process ends; that is
- * enough for graphs of about 4 M oriented edges (the 960 x 960 grid needs
- * 75 MB).  A larger graph returns STC_E_TOO_LARGE.  Same counts as
+ * counting pass's buffers (control block, edge and vertex lists, scan
+ * levels) come from one static device array of 128 MiB that the library
+ * holds on every GPU it runs on until the process ends; that is enough for
+ * graphs of about 4 M oriented edges (the 960 x 960 grid needs 72 MB).  A larger graph returns STC_E_TOO_LARGE.  Same counts as
  * stc_count_prepared in every mode, classes included (stc_last_classes);
  * per-kernel edge and byte statistics read zero.  Every call on one GPU
  * shares the array and keeps what the last call left there: calls that

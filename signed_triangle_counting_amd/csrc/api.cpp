@@ -502,6 +502,14 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
 
 }  // extern "C"
 
+// Dispatch parameters to the device for the next count: the host's copy,
+// plus whether every class-11 target list of the prepared graph fits the
+// current bitmap window (binning then needs no per-edge pass for pull).
+static hipError_t upload_params(stc_ctx *c) {
+  c->hp.pull_c11_fits = c->br.max_span_c11 <= (1u << c->hp.tune.bitmap_span_log2) ? 1u : 0u;
+  return hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+}
+
 // Binning + counting of the prepared graph.  cbuf: the buffers to count
 // with (null: the context's own, grown on first use; else borrowed views of
 // the static scratch, for which no per-kernel statistics are kept).
@@ -514,7 +522,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   c->hp.tune.allow_pull = 1;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   graph_features(c->g.n, c->g.m, 1, c->hp.graph_feat);
-  hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+  hipError_t e = upload_params(c);
   if (e != hipSuccess) return hip_fail(c, e, "params upload");
   // The event marks between the count's kernels (binning and per-kernel
   // times) are recorded in the first half of the runs only; with repeats,
@@ -827,7 +835,7 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
   c->hp.tune.allow_pull = 1;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
-  hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+  hipError_t e = upload_params(c);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
   const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
   std::vector<float> times, bins, kt[kKernels];
@@ -919,7 +927,7 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
   c->hp.mode = dispatch;
   c->hp.tune.allow_pull = 1;
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
-  hipError_t e = hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
+  hipError_t e = upload_params(c);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
   const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
   // eager run (allocates every buffer; its counts are what each replay must

@@ -327,31 +327,26 @@ __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uin
 }
 
 // ---- binning ----------------------------------------------------------------
-// Bucket histogram; every edge's bucket byte goes to bk8 (kNoBucket when
-// the edge cannot close a triangle) so the scatter pass need not recompute it.
-__global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl,
-                                                  const DispatchParams *P,
-                                                  uint8_t *__restrict__ bk8) {
+// Bucket histogram from the in-edge records, read in order: each record holds
+// its edge's suffix length (the source half of the bucket) and its target's
+// out-degree class (the target half), so no edge needs a gather.  A pass in
+// edge order gathered the target's class byte per edge, and was bound by the
+// L2's request rate (RMAT-22: 0.32 ms for 64 M edges).
+__global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
   __shared__ uint32_t h[kBuckets];
   const int tid = threadIdx.x;
   h[tid] = 0;
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
-    const bool in = e < g.m;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + tid; j < mround; j += stride) {
     bool valid = false;
     uint32_t bk = 0;
-    if (in) {
-      const uint32_t a = g.src[e], b = g.col[e] >> 1;
-      const uint32_t da = g.off[a + 1] - (uint32_t)e - 1u;
-      // the target's class: one byte gathered per edge that has a suffix (an
-      // edge with da = 0 is skipped whatever its target, and on graphs whose
-      // class array outgrows L2 the gather is a random line fetch)
-      const uint32_t cb = da ? g.dcls[b] : 0xFFu;
-      valid = da && cb != 0xFFu;
+    if (j < g.m) {
+      const uint32_t w = g.tin[j].w;
+      const uint32_t da = in_len(w), cb = in_cls(w);
+      valid = da && cb != kNoClass;
       if (valid) bk = (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb;
-      bk8[e] = valid ? (uint8_t)bk : kNoBucket;
     }
     uint32_t rank, cnt;
     wave_match8(bk, valid, rank, cnt);
@@ -359,6 +354,15 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl,
   }
   __syncthreads();
   if (h[tid]) atomicAdd(&ctrl->bucket_cnt[tid], h[tid]);
+}
+
+// Bucket of oriented edge e (kNoBucket when it cannot close a triangle), in
+// edge order: the target's class is gathered only for an edge with a suffix.
+__device__ __forceinline__ uint32_t edge_bucket(const DevCsr &g, uint64_t e, uint32_t a) {
+  const uint32_t da = g.off[a + 1] - (uint32_t)e - 1u;
+  const uint32_t cb = da ? g.dcls[g.col[e] >> 1] : 0xFFu;
+  return da && cb != 0xFFu ? (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb
+                           : kNoBucket;
 }
 
 // The strategy table from the bucket histogram (one workgroup, thread =
@@ -383,21 +387,20 @@ __device__ __forceinline__ uint32_t lds_append(bool pred, uint32_t *s_len) {
   return base + (uint32_t)__popcll(mask & lanemask_lt());
 }
 
-// Strategy of every edge from its stored bucket byte; merge edges go to the
+// Strategy of every edge from its bucket; merge edges go to the
 // workgroup's list segments, hash and bitmap edges flag their source vertex
 // (bit 0 hash, bit 1 bitmap).  Edges are sorted by source, so the lanes of one
 // source form runs: each run's first lane sets the run's flag bits with one
 // atomic, only when they are not set yet.  Pull edges need nothing here: the
 // vertex-list pass finds their targets from each target's suffix-class mask
 // (a per-edge flag on the target cost one device-scope atomic per pull edge:
-// 23 ms of a 92 ms batch of 605 M edges).  An edge whose bucket is pull with
-// a target class that always fits (the common case) is settled from its
-// bucket byte alone, without the list offsets; and a table whose buckets are
-// all pull (of such classes) or skipped leaves nothing to scatter at all, so
-// the workgroups return at once.
+// 23 ms of a 92 ms batch of 605 M edges).  A table whose buckets are all
+// skipped or pull of target classes that always fit (class <= 10, or 11 when
+// the prepared graph's class-11 lists all fit the window) leaves nothing to
+// scatter, so the workgroups return at once without reading an edge; else an
+// edge that such a bucket settles needs no list offsets.
 __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
                                                      const DispatchParams *P,
-                                                     const uint8_t *__restrict__ bk8,
                                                      uint32_t *__restrict__ lane_seg,
                                                      uint32_t *__restrict__ wave_seg,
                                                      uint32_t seg,
@@ -414,8 +417,11 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
     tune = P->tune;
     s_len[0] = s_len[1] = 0;
   }
+  // pull of a target class whose every list fits the pull kernels
+  const uint32_t fit_cls = kPullMaxClass + (P->pull_c11_fits ? 1u : 0u);
   // a bucket that lists merge edges or flags sources, or may demote pull
-  const bool other = mine != S_SKIP && (mine != S_PULL || tid % kBucketSide > kPullMaxClass);
+  const bool other = mine != S_SKIP && (mine != S_PULL || tid % kBucketSide > (int)fit_cls ||
+                                        !P->tune.allow_pull);
   if (!__syncthreads_or(other)) {
     if (tid == 0) blk_cnt[blockIdx.x] = blk_cnt[gridDim.x + blockIdx.x] = 0u;
     return;
@@ -428,11 +434,11 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
     uint8_t s = S_SKIP;
     uint32_t a = 0xFFFFFFFFu;
     if (e < g.m) {
-      const uint32_t bk = bk8[e];
+      a = g.src[e];
+      const uint32_t bk = edge_bucket(g, e, a);
       if (bk != kNoBucket) {
         s = table[bk];
-        if (s != S_PULL || (int)(bk % kBucketSide) > kPullMaxClass || !tune.allow_pull) {
-          a = g.src[e];
+        if (s != S_PULL || bk % kBucketSide > fit_cls || !tune.allow_pull) {
           const uint32_t la = g.off[a], ea = g.off[a + 1];
           s = bucket_strategy(
               table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
@@ -450,9 +456,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
     if (s == S_MERGE_WAVE) wave_out[p] = (uint32_t)e;
     const uint64_t hb = __ballot(s == S_HASH), bb = __ballot(s == S_BITMAP);
     if ((hb | bb) == 0ull) continue;  // wave-uniform: no source to flag
-    // runs of equal sources: heads, and each head's run as a lane mask (a
-    // is loaded for every edge of the wave here)
-    if (e < g.m) a = g.src[e];
+    // runs of equal sources: heads, and each head's run as a lane mask
     const uint32_t prev = (uint32_t)__shfl_up((int)a, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || prev != a);
     if ((heads >> lane) & 1u) {
@@ -723,9 +727,7 @@ __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams
 // vertex lists, added by k_list_stats: the staged list and the metadata of
 // every edge the item visits.
 __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
-                                                    const DispatchParams *P,
-                                                    const uint8_t *__restrict__ bk8,
-                                                    uint32_t batch) {
+                                                    const DispatchParams *P, uint32_t batch) {
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
   __shared__ unsigned long long s_e[10], s_w[10], s_b[10];
@@ -736,7 +738,7 @@ __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < g.m; e += stride) {
-    const uint32_t bk = bk8[e];
+    const uint32_t bk = edge_bucket(g, e, g.src[e]);
     uint32_t slot = 9, words = 0, bytes = 0;
     if (bk != kNoBucket) {
       const uint32_t a = g.src[e], b = g.col[e] >> 1;
@@ -1617,9 +1619,9 @@ struct PullMeta {
   __device__ __forceinline__ void operator()(uint32_t j, uint32_t &w, uint32_t &pb, uint32_t &eb,
                                              uint32_t &x) const {
     const InEdge r = g.tin[j];
-    w = r.len_neg & 1u;
+    w = r.w & 1u;
     pb = r.pos;
-    eb = r.pos + (r.len_neg >> 1);
+    eb = r.pos + in_len(r.w);
     x = g.pv ? g.src[r.pos - 1u] : 0u;
   }
 };
@@ -1969,9 +1971,9 @@ __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, u
   // the first metadata batch's records are the ones fetched above
   const auto meta = [&](uint32_t j, uint32_t &mw, uint32_t &pb, uint32_t &eb, uint32_t &x) {
     const InEdge r = j - j0 < 64u ? r0 : g.tin[j];
-    mw = r.len_neg & 1u;
+    mw = r.w & 1u;
     pb = r.pos;
-    eb = r.pos + (r.len_neg >> 1);
+    eb = r.pos + in_len(r.w);
     x = 0u;
   };
   const auto keep = [&](uint32_t, uint32_t pb, uint32_t eb) {
@@ -2461,7 +2463,7 @@ struct BinLayout {
   int bin_grid;
   uint32_t seg, ntiles;
   uint64_t seg_words, vstride;
-  size_t lists_bytes, blk_bytes, bk8_bytes, vflag_bytes, vlist_bytes;
+  size_t lists_bytes, blk_bytes, vflag_bytes, vlist_bytes;
 };
 
 BinLayout bin_layout(uint32_t n, uint64_t m, int num_cus) {
@@ -2475,7 +2477,6 @@ BinLayout bin_layout(uint32_t n, uint64_t m, int num_cus) {
   b.vstride = vlist_stride(n, m);
   b.lists_bytes = (2 * b.seg_words + 2 * (m ? m : 1)) * 4;
   b.blk_bytes = (4 * (uint64_t)b.bin_grid + 8) * 4;
-  b.bk8_bytes = m ? m : 1;
   b.vflag_bytes = ((uint64_t)n + 1) * 4;
   b.vlist_bytes = ((kVLists + 2) * b.vstride + 2 * kVLists * (uint64_t)b.ntiles + 8) * 4;
   return b;
@@ -2506,7 +2507,6 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   STC_TRY(cb.gcount.ensure(sizeof(Ctrl)));
   STC_TRY(cb.lists.ensure(lay.lists_bytes));
   STC_TRY(cb.blk.ensure(lay.blk_bytes));
-  STC_TRY(cb.bk8.ensure(lay.bk8_bytes));
   if (cb.vflag.bytes < lay.vflag_bytes) {
     STC_TRY(cb.vflag.ensure(lay.vflag_bytes));
     hipLaunchKernelGGL(k_fill_words, dim3(grid_for((uint64_t)n + 1, 256, 8192)), dim3(256),
@@ -2528,12 +2528,11 @@ hipError_t enqueue_binning(const DevCsr &g, const DispatchParams *dp, CountBuffe
   hipLaunchKernelGGL(k_fill_words, dim3(8), dim3(256), 0, st, (uint32_t *)ctrl,
                      (uint64_t)(sizeof(Ctrl) / 4), 0u);
   STC_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_bin_hist, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp,
-                     cb.bk8.as<uint8_t>());
+  hipLaunchKernelGGL(k_bin_hist, dim3(bin_grid), dim3(256), 0, st, g, ctrl);
   hipLaunchKernelGGL(k_bin_table, dim3(1), dim3(kBuckets), 0, st, ctrl, dp);
   STC_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_bin_scatter, dim3(bin_grid), dim3(256), 0, st, g, ctrl, dp,
-                     cb.bk8.as<uint8_t>(), lane_seg, wave_seg, seg, blk_cnt, vf);
+                     lane_seg, wave_seg, seg, blk_cnt, vf);
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(blk_cnt, blk_pos, 2 * (uint64_t)bin_grid, cb.scan, st));
   hipLaunchKernelGGL(k_list_compact, dim3(2 * bin_grid), dim3(256), 0, st, lane_seg, wave_seg,
@@ -2582,12 +2581,12 @@ bool carve_count_buffers(void *base, size_t bytes, uint32_t n, uint64_t m, int n
   for (int l = 0; l < 4; l++)
     level[l] = std::max<size_t>(256, std::max(scan_level_bytes(scan_n[0], l),
                                               scan_level_bytes(scan_n[1], l)));
-  const size_t parts[] = {sizeof(Ctrl),      lay.lists_bytes, lay.blk_bytes, lay.bk8_bytes,
-                          lay.vflag_bytes,   lay.vlist_bytes, level[0],      level[1],
-                          level[2],          level[3]};
-  DevBuf *views[] = {&cb.gcount, &cb.lists,         &cb.blk,           &cb.bk8,
-                     &cb.vflag,  &cb.vlist,         &cb.scan.level[0], &cb.scan.level[1],
-                     &cb.scan.level[2], &cb.scan.level[3]};
+  const size_t parts[] = {sizeof(Ctrl),    lay.lists_bytes, lay.blk_bytes, lay.vflag_bytes,
+                          lay.vlist_bytes, level[0],        level[1],      level[2],
+                          level[3]};
+  DevBuf *views[] = {&cb.gcount, &cb.lists,         &cb.blk,           &cb.vflag,
+                     &cb.vlist,  &cb.scan.level[0], &cb.scan.level[1], &cb.scan.level[2],
+                     &cb.scan.level[3]};
   size_t off = 0;
   for (size_t b : parts) off += (b + 255) & ~(size_t)255;
   if (need) *need = off;
@@ -2683,7 +2682,7 @@ hipError_t enqueue_count(const DevCsr &g, const DispatchParams &hp,
 hipError_t enqueue_stats(const DevCsr &g, const DispatchParams *dp, CountBuffers &cb,
                          int num_cus, bool batch, hipStream_t st) {
   hipLaunchKernelGGL(k_edge_stats, dim3(grid_for(g.m, 256, num_cus * 8)), dim3(256), 0, st, g,
-                     cb.gcount.as<Ctrl>(), dp, cb.bk8.as<uint8_t>(), batch ? 1u : 0u);
+                     cb.gcount.as<Ctrl>(), dp, batch ? 1u : 0u);
   STC_TRY(hipGetLastError());
   // the vertex lists of the run just counted (the buffers binning filled)
   BinOut L;

@@ -632,7 +632,8 @@ __global__ void k_tkeys(const uint32_t *__restrict__ col, const uint32_t *__rest
 }
 
 // In-edge records of the sorted keys (edge e = (u -> v): the part of N+(u)
-// after v and the edge's sign), in-degrees, and each target's suffix-class
+// after v, the edge's sign and v's out-degree class), in-degrees, and each
+// target's suffix-class
 // mask: bit c of smask[v] is set when some in-edge's part after v is
 // non-empty and of class min(ceil_log2 len, 15) -- the source side of the
 // binning bucket, so a counting pass can tell from the target alone whether
@@ -642,7 +643,8 @@ __global__ void k_tkeys(const uint32_t *__restrict__ col, const uint32_t *__rest
 // touching lane 0 or lane 63 may continue in a neighbouring wave and is
 // added (or or-ed) atomically into the zeroed arrays.
 __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m, uint32_t n,
-                         const uint32_t *__restrict__ rec, InEdge *__restrict__ tin,
+                         const uint32_t *__restrict__ rec, const uint8_t *__restrict__ dcls,
+                         InEdge *__restrict__ tin,
                          uint32_t *__restrict__ tdeg, uint32_t *__restrict__ smask,
                          uint32_t *__restrict__ err) {
   const uint32_t lane = __lane_id();
@@ -662,7 +664,8 @@ __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m, uint32_t 
       }
       const uint32_t r = rec[e];
       const uint32_t len = r >> 1;
-      tin[i] = InEdge{e + 1u, r};
+      const uint32_t vc = v < n ? dcls[v] : 0xFFu;
+      tin[i] = InEdge{e + 1u, r | ((vc == 0xFFu ? kNoClass : vc) << 26)};
       const uint32_t c = len <= 1 ? 0u : 32u - (uint32_t)__clz(len - 1u);
       cbit = len ? 1u << min(c, 15u) : 0u;
     }
@@ -697,25 +700,33 @@ __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m, uint32_t 
 // and the maximum out-degree, per vertex instead of per edge: v is the source
 // of d+(v) edges and the target of din(v) edges, each of which counts d+(v),
 // so the sum is sum_v d+(v) * (d+(v) + din(v)).
-__global__ void k_balg(const uint32_t *__restrict__ off, const uint32_t *__restrict__ tin_off,
-                       uint32_t n, unsigned long long *__restrict__ words,
-                       uint32_t *__restrict__ maxout) {
+// Merge-model words, largest out-degree, and the largest rank span of an
+// out-list of 1,025..2,048 entries (class 11: whether such a target list
+// fits the pull kernels' bitmap window depends on its span).
+__global__ void k_balg(const uint32_t *__restrict__ off, const uint32_t *__restrict__ col,
+                       const uint32_t *__restrict__ tin_off, uint32_t n,
+                       unsigned long long *__restrict__ words, uint32_t *__restrict__ maxout,
+                       uint32_t *__restrict__ maxspan) {
   unsigned long long acc = 0;
-  uint32_t mo = 0;
+  uint32_t mo = 0, ms = 0;
   for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t dout = off[v + 1] - off[v], din = tin_off[v + 1] - tin_off[v];
     acc += (unsigned long long)dout * ((unsigned long long)dout + din);
     mo = max(mo, dout);
+    if (dout > 1024u && dout <= 2048u)
+      ms = max(ms, (col[off[v + 1] - 1u] >> 1) - (col[off[v]] >> 1) + 1u);
   }
   // wave reduce then one atomic per wave
   for (int d = 32; d > 0; d >>= 1) {
     acc += __shfl_down(acc, d, 64);
     mo = max(mo, (uint32_t)__shfl_down(mo, d, 64));
+    ms = max(ms, (uint32_t)__shfl_down(ms, d, 64));
   }
   if (__lane_id() == 0) {
     atomicAdd(words, acc);
     atomicMax(maxout, mo);
+    if (ms) atomicMax(maxspan, ms);
   }
 }
 
@@ -850,7 +861,8 @@ hipError_t build_begin(const BuildInput &in, BuildBuffers &bb, std::string &err,
     return hipErrorInvalidValue;
   }
   STC_TRY(bb.misc.ensure(4096));
-  // misc: [0] bad id, [1] maxdeg, [2] maxout, [4..5] words, [6] out-of-range sorted id
+  // misc: [0] bad id, [1] maxdeg, [2] maxout, [3] largest class-11 span, [4..5] words,
+  // [6] out-of-range sorted id
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, bb.misc.as<uint32_t>(), (uint64_t)16,
                      0u);
   // off/fdeg/etc. sized even for m == 0
@@ -999,12 +1011,14 @@ hipError_t build_finish(const BuildInput &in, BuildBuffers &bb, BuildResult &res
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1, T)), dim3(T), 0, st,
                      bb.smask.as<uint32_t>(), (uint64_t)n + 1, 0u);
   hipLaunchKernelGGL(k_tsplit, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m, n, rec,
-                     bb.tin.as<InEdge>(), deg, bb.smask.as<uint32_t>(), id_err);
+                     bb.dcls.as<uint8_t>(), bb.tin.as<InEdge>(), deg, bb.smask.as<uint32_t>(),
+                     id_err);
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(deg, bb.tin_off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
   // algorithmic bytes and max out-degree
   hipLaunchKernelGGL(k_balg, dim3(grid_for(n, T, 4096)), dim3(T), 0, st, bb.off.as<uint32_t>(),
-                     bb.tin_off.as<uint32_t>(), n, (unsigned long long *)(misc + 4), misc + 2);
+                     bb.col.as<uint32_t>(), bb.tin_off.as<uint32_t>(), n,
+                     (unsigned long long *)(misc + 4), misc + 2, misc + 3);
   STC_TRY(hipGetLastError());
   uint32_t fin[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   STC_TRY(hipMemcpyAsync(fin, misc, sizeof fin, hipMemcpyDeviceToHost, st));
@@ -1017,6 +1031,7 @@ hipError_t build_finish(const BuildInput &in, BuildBuffers &bb, BuildResult &res
   unsigned long long words = 0;
   std::memcpy(&words, fin + 4, 8);
   res.max_out = mo;
+  res.max_span_c11 = fin[3];
   res.b_alg = 4.0 * (double)words + 4.0 * (double)m + 8.0 * ((double)n + 1.0);
   return hipSuccess;
 }

@@ -89,14 +89,24 @@ struct DispatchParams {
   uint8_t host_table[kBuckets];  // D_TABLE
   float graph_feat[3];           // log2 (n/k + 1), log2 (m/k + 1), log2 k for k graphs
                                  // counted together (k = 1: one graph)
+  // 1 when every target list of class 11 (1,024 < d+ <= 2,048 entries) spans
+  // at most the workgroup bitmap window, so a pull bucket of class <= 11 never
+  // demotes an edge (from the prepared graph's largest such span)
+  uint32_t pull_c11_fits;
 };
 
 // One in-edge e = (u -> v) of the transposed index: the part of N+(u) after
-// v is col[pos, pos + (len_neg >> 1)) with pos = e + 1; bit 0 of len_neg is
-// the edge's sign.
+// v is col[pos, pos + len) with pos = e + 1.  The word w holds the edge's sign
+// in bit 0, len in bits 1..25 and v's out-degree class min(ceil_log2 d+(v),
+// 15) in bits 26..30 (kNoClass when d+(v) = 0), so a record alone gives the
+// edge's binning bucket.  len <= d+(u) <= sqrt(2 m) < 2^17: every out-
+// neighbour of u has degree >= deg(u) >= d+(u) under the (degree, id) order.
 struct InEdge {
-  uint32_t pos, len_neg;
+  uint32_t pos, w;
 };
+constexpr uint32_t kNoClass = 31;
+__host__ __device__ inline uint32_t in_len(uint32_t w) { return (w >> 1) & ((1u << 25) - 1u); }
+__host__ __device__ inline uint32_t in_cls(uint32_t w) { return w >> 26; }
 
 // Device view of the oriented CSR (rank space).
 struct DevCsr {
@@ -231,6 +241,7 @@ struct BuildResult {
   uint32_t n = 0;
   uint64_t m_in = 0, m = 0;
   uint32_t max_deg = 0, max_out = 0;
+  uint32_t max_span_c11 = 0;  // largest rank span of an out-list of 1,025..2,048 entries
   double b_alg = 0;
 };
 // Builds the rank-space oriented CSR (and the transposed index) on stream st
@@ -245,7 +256,7 @@ hipError_t build_finish(const BuildInput &in, BuildBuffers &bb, BuildResult &res
                         std::string &err, hipStream_t st);
 
 struct CountBuffers {
-  DevBuf gcount, lists, blk, vflag, vlist, bk8, pv, pv_ids;
+  DevBuf gcount, lists, blk, vflag, vlist, pv, pv_ids;
   ScanScratch scan;
 };
 

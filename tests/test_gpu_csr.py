@@ -64,17 +64,20 @@ def test_device_csr_equals_oracle_forward_csr(ctx, case):
     want_col = (dst_r[order] << 1) | (S[order] < 0)
     assert (a["col"].astype(np.int64) == want_col).all()
     # transposed index: per target rank, its in-edges e ascending, as
-    # (e + 1, (|N+(u) after v| << 1) | neg)
+    # (e + 1, (class of d+(v) << 26) | (|N+(u) after v| << 1) | neg), the
+    # class min(ceil_log2 d+(v), 15), or 31 when d+(v) = 0
     src_e, dst_e = src_r[order], dst_r[order]
     e = np.arange(m, dtype=np.int64)
     suffix = want_off[src_e + 1] - e - 1
     by_t = np.lexsort((e, dst_e))
     want_tin_off = np.concatenate([[0], np.cumsum(np.bincount(dst_e, minlength=n))])
     assert (a["tin_off"].astype(np.int64) == want_tin_off).all()
-    want_tin = np.stack([e[by_t] + 1, (suffix[by_t] << 1) | (S[order][by_t] < 0)], axis=1)
+    want_dcls = np.where(fdr == 0, 0xFF, np.minimum(clog2(fdr), 15))
+    tcls = np.where(fdr[dst_e] == 0, 31, np.minimum(clog2(fdr[dst_e]), 15))
+    want_tin = np.stack([e[by_t] + 1, (tcls[by_t] << 26) | (suffix[by_t] << 1) | (S[order][by_t] < 0)],
+                        axis=1)
     assert (a["tin"].astype(np.int64) == want_tin).all()
     # per-rank out-degree class and suffix-class mask (binning's two halves)
-    want_dcls = np.where(fdr == 0, 0xFF, np.minimum(clog2(fdr), 15))
     assert (a["dcls"].astype(np.int64) == want_dcls).all()
     bits = np.where(suffix > 0, np.left_shift(1, np.minimum(clog2(suffix), 15)), 0)
     want_smask = np.zeros(n, np.int64)
