@@ -332,6 +332,11 @@ __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uin
 // out-degree class (the target half), so no edge needs a gather.  A pass in
 // edge order gathered the target's class byte per edge, and was bound by the
 // L2's request rate (RMAT-22: 0.32 ms for 64 M edges).
+// Each thread takes kHistUnroll records per round, loaded before any is
+// counted, so a wave keeps that many loads in flight (one per round left
+// the pass latency-bound at under half of HBM's rate).
+constexpr int kHistUnroll = 4;
+
 __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
   __shared__ uint32_t h[kBuckets];
   const int tid = threadIdx.x;
@@ -339,18 +344,25 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + tid; j < mround; j += stride) {
-    bool valid = false;
-    uint32_t bk = 0;
-    if (j < g.m) {
-      const uint32_t w = g.tin[j].w;
-      const uint32_t da = in_len(w), cb = in_cls(w);
-      valid = da && cb != kNoClass;
-      if (valid) bk = (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb;
+  for (uint64_t j0 = (uint64_t)blockIdx.x * blockDim.x + tid; j0 < mround;
+       j0 += kHistUnroll * stride) {
+    uint32_t w[kHistUnroll];
+#pragma unroll
+    for (int k = 0; k < kHistUnroll; k++) {
+      const uint64_t j = j0 + k * stride;
+      w[k] = j < g.m ? g.tin[j].w : (kNoClass << 26);
     }
-    uint32_t rank, cnt;
-    wave_match8(bk, valid, rank, cnt);
-    if (valid && rank == 0) atomicAdd(&h[bk], cnt);
+#pragma unroll
+    for (int k = 0; k < kHistUnroll; k++) {
+      if (j0 + k * stride >= mround) break;  // wave-uniform
+      const uint32_t da = in_len(w[k]), cb = in_cls(w[k]);
+      const bool valid = da && cb != kNoClass;
+      const uint32_t bk =
+          valid ? (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb : 0u;
+      uint32_t rank, cnt;
+      wave_match8(bk, valid, rank, cnt);
+      if (valid && rank == 0) atomicAdd(&h[bk], cnt);
+    }
   }
   __syncthreads();
   if (h[tid]) atomicAdd(&ctrl->bucket_cnt[tid], h[tid]);
@@ -613,6 +625,7 @@ __global__ __launch_bounds__(256) void k_vcount(DevCsr g, const Ctrl *ctrl,
   const bool any_pull = any_pull_in(s_pull);
   uint32_t cnt[kVLists] = {0, 0, 0, 0, 0, 0};
   const uint32_t base = blockIdx.x * kVTile;
+#pragma unroll 4  // rounds are independent: their loads overlap
   for (uint32_t r = 0; r < kVTile; r += 256) {
     uint32_t c[kVLists];
     uint32_t f;
@@ -663,11 +676,17 @@ __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams
   __syncthreads();
   const bool any_pull = any_pull_in(s_pull);
   const uint32_t base = b * kVTile;
+  // the next round's lists are found while this round's are written (their
+  // loads are in flight across this round's barriers)
+  uint32_t cn[kVLists], fn;
+  vertex_lists_of(base + tid, g, vflag, s_pull, any_pull, tune, cn, fn);
   for (uint32_t r = 0; r < kVTile; r += 256) {
     const uint32_t v = base + r + tid;
     uint32_t c[kVLists];
-    uint32_t f;
-    vertex_lists_of(v, g, vflag, s_pull, any_pull, tune, c, f);
+#pragma unroll
+    for (int L = 0; L < kVLists; L++) c[L] = cn[L];
+    const uint32_t f = fn;
+    if (r + 256 < kVTile) vertex_lists_of(v + 256, g, vflag, s_pull, any_pull, tune, cn, fn);
     uint32_t incl[kVLists];
 #pragma unroll
     for (int L = 0; L < kVLists; L++) {
