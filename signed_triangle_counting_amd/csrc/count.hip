@@ -39,6 +39,9 @@ constexpr uint32_t kBitmapMaxDeg = 2048;
 // Entries of a workgroup-bitmap source list that fall below its rank window
 // are kept as a sorted LDS array of at most kLowCap words.
 constexpr uint32_t kLowCap = 1024;
+// Zero words after every LDS bitmap: a probe past the window reads the first
+// of them (four keep what follows 16-byte aligned).
+constexpr uint32_t kGuard = 4;
 constexpr int kVertexBlock = 256;   // threads per workgroup of the hash kernel
 constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kernel
 
@@ -1188,29 +1191,30 @@ __device__ __forceinline__ uint32_t low_lookup(const uint32_t *__restrict__ L, u
 // bits per rank, one-hot by sign: rank lo + r with sign s sets bit 2r + s.
 // The sorted array L[0, nlow) holds the entries below lo.  With d2 = y - 2 lo
 // = 2 (rank - lo) + nb, the word is d2 >> 5, bit d2 & 31 is the same-sign bit
-// and bit (d2 ^ 1) & 31 the opposite-sign bit (one bit-field extract each),
-// and the window test is d2 < 2 span; padding (kEmpty) and ranks above the
-// window fail it.  A 64-element group with a
+// and bit (d2 ^ 1) & 31 the opposite-sign bit (one bit-field extract each).
+// The word index is clamped to zw, the word after the window's last: no bit
+// at or past 2 span is ever set (the window ends at the list's highest rank,
+// and staging sets no other bits), and word zw is zero too (a word of the
+// bitmap past the list, or the guard after a full window), so padding
+// (kEmpty) and ranks above or below the window miss without a window test.
+// A 64-element group with a
 // target below lo (a prefix of the sorted N+(v), so rare) also searches L for
 // those lanes.  A target below lo wraps d2 to at least 2^32 - 2 lo > 2^21 >=
 // 2 span, and kEmpty - 1 - 2 lo exceeds 2 span too, because the API admits at most
 // 2^31 - 2^20 vertices (ranks).  LOW = false is the probe of a list with no
 // entries below the window (nlow = 0), which skips the low test entirely.
-// The word is read whatever the window test says, at an index clamped to the
-// window's last word (lastw), and zeroed after: with no branch around the
-// read, the reads of an unrolled group go out together instead of each
-// waiting for the one before.
+// No branch sits around the read, so the reads of an unrolled group go out
+// together instead of each waiting for the one before.
 template <bool LOW>
 struct BitmapProbe {
   const uint32_t *B;
-  uint32_t lo2, span2, lastw;
+  uint32_t lo2, zw;
   const uint32_t *L;
   uint32_t nlow;
   __device__ __forceinline__ void operator()(uint32_t y, uint32_t s, uint32_t &same,
                                              uint32_t &diff) const {
     const uint32_t d2 = y - lo2;
-    uint32_t w = B[min(d2 >> 5, lastw)];
-    w = d2 < span2 ? w : 0u;
+    const uint32_t w = B[min(d2 >> 5, zw)];
     // v_bfe_u32 reads bits [4:0] of its offset operand: d2 & 31 needs no mask;
     // s flips bit 0, the sign half of the rank's two bits
     const uint32_t ds = d2 ^ s;
@@ -1738,11 +1742,12 @@ struct Staged {
   template <class F>
   __device__ __forceinline__ void with_probe(const uint32_t *B, const uint32_t *L,
                                              const F &f) const {
-    const uint32_t lo2 = base << 1, span2 = (hi - base + 1u) << 1;
+    // zw: the word after the one holding the window's highest bit
+    const uint32_t lo2 = base << 1, zw = (((hi - base + 1u) << 1) + 31u) >> 5;
     if (nlow)
-      f(BitmapProbe<true>{B, lo2, span2, (span2 - 1u) >> 5, L, nlow});
+      f(BitmapProbe<true>{B, lo2, zw, L, nlow});
     else
-      f(BitmapProbe<false>{B, lo2, span2, (span2 - 1u) >> 5, L, 0u});
+      f(BitmapProbe<false>{B, lo2, zw, L, 0u});
   }
 };
 
@@ -2123,7 +2128,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                          const uint32_t *__restrict__ vlist,
                                                          uint32_t bitmap_words) {
   extern __shared__ uint32_t B[];
-  uint32_t *L = B + bitmap_words;
+  uint32_t *L = B + bitmap_words + kGuard;
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
@@ -2131,7 +2136,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
   if (len == 0) return;  // the whole grid: no vertex has this kernel's edges
   if (threadIdx.x < kBuckets) table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
-  for (uint32_t k = threadIdx.x; k < bitmap_words; k += blockDim.x) B[k] = 0u;
+  for (uint32_t k = threadIdx.x; k < bitmap_words + kGuard; k += blockDim.x) B[k] = 0u;
   QueueCursor cur{blockIdx.x & 7u, 0u};
   Acc<CLS> acc;
   uint32_t first;
@@ -2163,9 +2168,9 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
   const uint32_t lane = __lane_id();
   table[threadIdx.x] = ctrl->table[threadIdx.x];
   if (threadIdx.x == 0) tune = P->tune;
-  uint32_t *B = lds + wave * (window_words + kWaveMaxDeg);
-  uint32_t *L = B + window_words;
-  for (uint32_t k = lane; k < window_words; k += 64) B[k] = 0u;
+  uint32_t *B = lds + wave * (window_words + kGuard + kWaveMaxDeg);
+  uint32_t *L = B + window_words + kGuard;
+  for (uint32_t k = lane; k < window_words + kGuard; k += 64) B[k] = 0u;
   __syncthreads();
   const uint32_t window = window_words * 16u;
   QueueCursor cur{blockIdx.x & 7u, 0u};
@@ -2231,11 +2236,12 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 // its in-edges come from its own graph; graph_of is indexed by rank; each
 // graph's ranks are one contiguous range).
 //   All LDS structures share one region R, kept zero between items.  Phase 1
-// uses the workgroup bitmap R[0, bitmap_words) with its low array after it,
+// uses the workgroup bitmap R[0, bitmap_words), kGuard zero words and the
+// low array after them,
 // or the workgroup hash table R[0, hash_slots).  In phase 2 wave w owns the
-// slot R[w S, (w + 1) S), S = window_words + kWaveMaxDeg + kPackWords:
-// its hash table or window bitmap at the start, the window's low array after
-// the window, and its packed-streaming records and start masks at the end
+// slot R[w S, (w + 1) S), S = window_words + kGuard + kWaveMaxDeg + kPackWords:
+// its hash table or window bitmap at the start, the window's guard words and
+// low array after the window, and its packed-streaming records and start masks at the end
 // (they need no clearing: phase 1, which needs zeros there, is over by then,
 // and the streaming clears the masks itself).  Bitmaps
 // clear what they set; tables and low arrays are cleared after use.
@@ -2332,22 +2338,24 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
       } else if (ALL && it < b3) {  // bitmap vertex
         const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[vstride + it - b2]);
         ga.to(per_graph, graph_of[a]);
-        bitmap_vertex_block<CLS>(g, a, R, R + bitmap_words, &s_nlow, table, tune, acc);
-        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
+        bitmap_vertex_block<CLS>(g, a, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
+                                 acc);
+        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
       } else {  // workgroup-pull chunk
         const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[4 * vstride + it - b3]);
         const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[6 * vstride + it - b3]);
         ga.to(per_graph, graph_of ? graph_of[v] : 0u);
-        pull_vertex_block<CLS>(g, j0, v, R, R + bitmap_words, &s_nlow, table, tune, acc);
-        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + k] = 0u;
+        pull_vertex_block<CLS>(g, j0, v, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
+                               acc);
+        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
       }
     }
   }
   // phase 2: wave items [merge-wave | wave hash | wave bitmap | wave pull]
   {
-    const uint32_t slot = window_words + kWaveMaxDeg + kPackWords;
+    const uint32_t slot = window_words + kGuard + kWaveMaxDeg + kPackWords;
     uint32_t *const W = R + wave * slot;
-    uint32_t *const LW = W + window_words;
+    uint32_t *const LW = W + window_words + kGuard;
     uint64_t *const HW = (uint64_t *)(LW + kWaveMaxDeg);
     const uint32_t window = window_words * 16u;
     const uint32_t w1 = nwv, w2 = w1 + nhw, w3 = w2 + nbw, total = w3 + nwp;
@@ -2432,10 +2440,10 @@ size_t hash_lds_bytes(const TuneParams &t) {
   return (size_t)4u << std::max(t.hash_max_log2 + 2, 6);
 }
 size_t bitmap_lds_bytes(const TuneParams &t) {
-  return (size_t)((1u << t.bitmap_span_log2) / 4u) + 4u * kLowCap;
+  return (size_t)((1u << t.bitmap_span_log2) / 4u) + 4u * (kGuard + kLowCap);
 }
 size_t wave_bitmap_lds_bytes(const TuneParams &t) {
-  return 4u * (size_t)(256 / 64) * ((1u << t.wave_window_log2) / 16u + kWaveMaxDeg);
+  return 4u * (size_t)(256 / 64) * ((1u << t.wave_window_log2) / 16u + kGuard + kWaveMaxDeg);
 }
 
 #define STC_TRY(expr)                \
@@ -2721,11 +2729,11 @@ hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *
 
 // Words of the batch kernel's shared LDS region (the largest structure).
 static size_t batch_region_words(const TuneParams &t) {
-  const size_t bitmap = (size_t)((1u << t.bitmap_span_log2) / 16u) + kLowCap;
+  const size_t bitmap = (size_t)((1u << t.bitmap_span_log2) / 16u) + kGuard + kLowCap;
   const size_t hash = (size_t)4u << t.hash_max_log2;
   // phase 2: one slot per wave (hash table or window + low array, then heads)
   const size_t wave_slot = std::max<size_t>(kWaveCap, (1u << t.wave_window_log2) / 16u) +
-                           kWaveMaxDeg + kPackWords;
+                           kGuard + kWaveMaxDeg + kPackWords;
   const size_t waves = (size_t)kWavesPerBatchBlock * wave_slot;
   return std::max(bitmap, std::max(hash, waves));
 }
