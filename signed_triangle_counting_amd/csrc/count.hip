@@ -252,20 +252,6 @@ __device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table, const Tun
 // Bucket byte of an edge that cannot close a triangle (da = 0 or db = 0).
 constexpr uint8_t kNoBucket = 0xFF;
 
-// Rank of this lane among lanes of the wave with the same 8-bit value.
-__device__ __forceinline__ void wave_match8(uint32_t d, bool valid, uint32_t &rank,
-                                            uint32_t &cnt) {
-  uint64_t peers = __ballot(valid);
-#pragma unroll
-  for (int b = 0; b < 8; b++) {
-    bool bit = (d >> b) & 1u;
-    uint64_t bb = __ballot(bit);
-    peers &= bit ? bb : ~bb;
-  }
-  rank = (uint32_t)__popcll(peers & lanemask_lt());
-  cnt = (uint32_t)__popcll(peers);
-}
-
 // Wave-aggregated append: returns this lane's slot in `list` (valid lanes).
 __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *len) {
   uint64_t mask = __ballot(pred);
@@ -338,13 +324,21 @@ __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uin
 // Each thread takes kHistUnroll records per round, loaded before any is
 // counted, so a wave keeps that many loads in flight (one per round left
 // the pass latency-bound at under half of HBM's rate).
+//   The workgroup keeps kHistCopies copies of each bucket's counter and lane
+// l adds to copy l % kHistCopies, so at most 8 lanes of a wave add to one
+// LDS word: the lanes of a wave often share a bucket (consecutive in-edges
+// of one target), and one counter per bucket would serialize them.  Grouping
+// the lanes by bucket first (an 8-round ballot match) cost about 55 vector
+// instructions per record round and left the pass bound by the vector unit.
 constexpr int kHistUnroll = 4;
+constexpr uint32_t kHistCopies = 8;
 
 __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
-  __shared__ uint32_t h[kBuckets];
+  __shared__ uint32_t h[kBuckets * kHistCopies];
   const int tid = threadIdx.x;
-  h[tid] = 0;
+  for (uint32_t k = tid; k < kBuckets * kHistCopies; k += 256) h[k] = 0;
   __syncthreads();
+  const uint32_t copy = __lane_id() % kHistCopies;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t j0 = (uint64_t)blockIdx.x * blockDim.x + tid; j0 < mround;
@@ -359,16 +353,17 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
     for (int k = 0; k < kHistUnroll; k++) {
       if (j0 + k * stride >= mround) break;  // wave-uniform
       const uint32_t da = in_len(w[k]), cb = in_cls(w[k]);
-      const bool valid = da && cb != kNoClass;
-      const uint32_t bk =
-          valid ? (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb : 0u;
-      uint32_t rank, cnt;
-      wave_match8(bk, valid, rank, cnt);
-      if (valid && rank == 0) atomicAdd(&h[bk], cnt);
+      if (da && cb != kNoClass) {
+        const uint32_t bk = (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb;
+        atomicAdd(&h[bk * kHistCopies + copy], 1u);
+      }
     }
   }
   __syncthreads();
-  if (h[tid]) atomicAdd(&ctrl->bucket_cnt[tid], h[tid]);
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t c = 0; c < kHistCopies; c++) sum += h[tid * kHistCopies + c];
+  if (sum) atomicAdd(&ctrl->bucket_cnt[tid], sum);
 }
 
 // Bucket of oriented edge e (kNoBucket when it cannot close a triangle), in
