@@ -74,8 +74,7 @@ int stc_last_table(const stc_ctx *ctx, uint8_t out[STC_BUCKETS]);
  * out-list offsets by rank; col[m] the lists, (rank << 1) | neg ascending by
  * rank; perm[n] the vertex id of each rank; tin_off[n + 1] and tin[2 m] the
  * transposed index, per target rank its in-edges e = (u -> v) in ascending
- * e as (e + 1, (c << 26) | (|N+(u) after v| << 1) | neg) pairs, c the target's
- * out-degree class (31 for an empty list); dcls[n] each rank's
+ * e as (e + 1, (|N+(u) after v| << 1) | neg) pairs; dcls[n] each rank's
  * out-degree class min(ceil_log2 d+, 15) (0xFF for an empty list); smask[n]
  * the suffix classes of its in-edges.  Replaces, for tests, the reference's
  * ForwardCSR (Multicore Implementations/graph_io.h:22-30,221-402), which

@@ -485,6 +485,7 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
   c->g.tin_off = c->bb.tin_off.as<uint32_t>();
   c->g.tin = c->bb.tin.as<InEdge>();
   c->g.dcls = c->bb.dcls.as<uint8_t>();
+  c->g.ebk = c->bb.ebk.as<uint16_t>();
   c->g.smask = c->bb.smask.as<uint32_t>();
   c->prepared = true;
   c->batch_ready = false;
@@ -810,6 +811,7 @@ static int prepare_batch(stc_ctx *c, const stc_edges *graphs, uint32_t k) {
   c->g.tin_off = c->bb.tin_off.as<uint32_t>();
   c->g.tin = c->bb.tin.as<InEdge>();
   c->g.dcls = c->bb.dcls.as<uint8_t>();
+  c->g.ebk = c->bb.ebk.as<uint16_t>();
   c->g.smask = c->bb.smask.as<uint32_t>();
   c->batch_ready = true;
   c->batch_k = k;

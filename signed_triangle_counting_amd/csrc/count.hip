@@ -249,9 +249,6 @@ __device__ __forceinline__ uint8_t edge_strategy(const uint8_t *table, const Tun
                          [&] { return span_of(col, pb, eb); }, bucket_of(da_eff, db));
 }
 
-// Bucket byte of an edge that cannot close a triangle (da = 0 or db = 0).
-constexpr uint8_t kNoBucket = 0xFF;
-
 // Wave-aggregated append: returns this lane's slot in `list` (valid lanes).
 __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *len) {
   uint64_t mask = __ballot(pred);
@@ -316,21 +313,18 @@ __device__ __forceinline__ bool queue_pop(uint32_t (*ctr)[32], uint32_t len, uin
 }
 
 // ---- binning ----------------------------------------------------------------
-// Bucket histogram from the in-edge records, read in order: each record holds
-// its edge's suffix length (the source half of the bucket) and its target's
-// out-degree class (the target half), so no edge needs a gather.  A pass in
-// edge order gathered the target's class byte per edge, and was bound by the
-// L2's request rate (RMAT-22: 0.32 ms for 64 M edges).
-// Each thread takes kHistUnroll records per round, loaded before any is
-// counted, so a wave keeps that many loads in flight (one per round left
-// the pass latency-bound at under half of HBM's rate).
+// Bucket histogram from the edge buckets the build stored (two bytes per
+// edge, in edge order, read as 16-byte groups of eight): no edge needs a
+// gather.  Read from the in-edge records (eight bytes per edge, the target's
+// class packed beside the suffix length) the pass moved four times the
+// bytes; gathering the target's class per edge was bound by the L2's
+// request rate (RMAT-22: 0.32 ms for 64 M edges).
 //   The workgroup keeps kHistCopies copies of each bucket's counter and lane
 // l adds to copy l % kHistCopies, so at most 8 lanes of a wave add to one
-// LDS word: the lanes of a wave often share a bucket (consecutive in-edges
-// of one target), and one counter per bucket would serialize them.  Grouping
-// the lanes by bucket first (an 8-round ballot match) cost about 55 vector
-// instructions per record round and left the pass bound by the vector unit.
-constexpr int kHistUnroll = 4;
+// LDS word: the lanes of a wave often share a bucket (the edges of one
+// source), and one counter per bucket would serialize them.  Grouping the
+// lanes by bucket first (an 8-round ballot match) cost about 55 vector
+// instructions per round and left the pass bound by the vector unit.
 constexpr uint32_t kHistCopies = 8;
 
 __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
@@ -339,24 +333,16 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
   for (uint32_t k = tid; k < kBuckets * kHistCopies; k += 256) h[k] = 0;
   __syncthreads();
   const uint32_t copy = __lane_id() % kHistCopies;
+  const uint4 *q = (const uint4 *)g.ebk;  // 8 edge buckets per group
+  const uint64_t ng = (g.m + 7) >> 3;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
-  for (uint64_t j0 = (uint64_t)blockIdx.x * blockDim.x + tid; j0 < mround;
-       j0 += kHistUnroll * stride) {
-    uint32_t w[kHistUnroll];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + tid; i < ng; i += stride) {
+    const uint4 x = q[i];
+    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-    for (int k = 0; k < kHistUnroll; k++) {
-      const uint64_t j = j0 + k * stride;
-      w[k] = j < g.m ? g.tin[j].w : (kNoClass << 26);
-    }
-#pragma unroll
-    for (int k = 0; k < kHistUnroll; k++) {
-      if (j0 + k * stride >= mround) break;  // wave-uniform
-      const uint32_t da = in_len(w[k]), cb = in_cls(w[k]);
-      if (da && cb != kNoClass) {
-        const uint32_t bk = (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb;
-        atomicAdd(&h[bk * kHistCopies + copy], 1u);
-      }
+    for (int k = 0; k < 8; k++) {
+      const uint32_t bk = (w[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+      if (bk != kNoEdgeBucket) atomicAdd(&h[bk * kHistCopies + copy], 1u);
     }
   }
   __syncthreads();
@@ -364,15 +350,6 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
 #pragma unroll
   for (uint32_t c = 0; c < kHistCopies; c++) sum += h[tid * kHistCopies + c];
   if (sum) atomicAdd(&ctrl->bucket_cnt[tid], sum);
-}
-
-// Bucket of oriented edge e (kNoBucket when it cannot close a triangle), in
-// edge order: the target's class is gathered only for an edge with a suffix.
-__device__ __forceinline__ uint32_t edge_bucket(const DevCsr &g, uint64_t e, uint32_t a) {
-  const uint32_t da = g.off[a + 1] - (uint32_t)e - 1u;
-  const uint32_t cb = da ? g.dcls[g.col[e] >> 1] : 0xFFu;
-  return da && cb != 0xFFu ? (uint32_t)(min(clog2(da), kBucketSide - 1) * kBucketSide) + cb
-                           : kNoBucket;
 }
 
 // The strategy table from the bucket histogram (one workgroup, thread =
@@ -442,13 +419,15 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
   const uint64_t mround = (g.m + 63) & ~63ull;  // keep waves converged
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < mround; e += stride) {
     uint8_t s = S_SKIP;
-    uint32_t a = 0xFFFFFFFFu;
     if (e < g.m) {
-      a = g.src[e];
-      const uint32_t bk = edge_bucket(g, e, a);
-      if (bk != kNoBucket) {
+      const uint32_t bk = g.ebk[e];
+      if (bk != kNoEdgeBucket) {
         s = table[bk];
-        if (s != S_PULL || bk % kBucketSide > fit_cls || !tune.allow_pull) {
+        // merge and skipped buckets keep their strategy, and so does pull of
+        // a target class that always fits; the others may be demoted
+        if (s == S_HASH || s == S_BITMAP ||
+            (s == S_PULL && (bk % kBucketSide > fit_cls || !tune.allow_pull))) {
+          const uint32_t a = g.src[e];
           const uint32_t la = g.off[a], ea = g.off[a + 1];
           s = bucket_strategy(
               table, tune, ea - la, [&] { return span_of(g.col, la, ea); },
@@ -466,6 +445,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(DevCsr g, Ctrl *ctrl,
     if (s == S_MERGE_WAVE) wave_out[p] = (uint32_t)e;
     const uint64_t hb = __ballot(s == S_HASH), bb = __ballot(s == S_BITMAP);
     if ((hb | bb) == 0ull) continue;  // wave-uniform: no source to flag
+    const uint32_t a = e < g.m ? g.src[e] : 0xFFFFFFFFu;
     // runs of equal sources: heads, and each head's run as a lane mask
     const uint32_t prev = (uint32_t)__shfl_up((int)a, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || prev != a);
@@ -755,9 +735,9 @@ __global__ __launch_bounds__(256) void k_edge_stats(DevCsr g, Ctrl *ctrl,
   __syncthreads();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + tid; e < g.m; e += stride) {
-    const uint32_t bk = edge_bucket(g, e, g.src[e]);
+    const uint32_t bk = g.ebk[e];
     uint32_t slot = 9, words = 0, bytes = 0;
-    if (bk != kNoBucket) {
+    if (bk != kNoEdgeBucket) {
       const uint32_t a = g.src[e], b = g.col[e] >> 1;
       const uint32_t la = g.off[a], ea = g.off[a + 1];
       const uint32_t db = g.off[b + 1] - g.off[b];

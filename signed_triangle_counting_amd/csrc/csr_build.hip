@@ -608,6 +608,29 @@ __global__ void k_dcls(const uint32_t *__restrict__ off, uint32_t n, uint8_t *__
   }
 }
 
+// Binning bucket of every oriented edge e = (u -> v), in edge order (the
+// count's binning pass then reads two bytes per edge instead of gathering
+// the target's class); the entries from m up to mp (m rounded up to 8) are
+// kNoEdgeBucket, so the pass reads whole 16-byte groups.
+__global__ void k_edge_buckets(const uint32_t *__restrict__ off, const uint32_t *__restrict__ col,
+                               const uint32_t *__restrict__ src,
+                               const uint8_t *__restrict__ dcls, uint64_t m, uint64_t mp,
+                               uint16_t *__restrict__ ebk) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < mp;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint16_t b = kNoEdgeBucket;
+    if (e < m) {
+      const uint32_t da = off[src[e] + 1] - (uint32_t)e - 1u;
+      const uint32_t vc = da ? dcls[col[e] >> 1] : 0xFFu;
+      if (vc != 0xFFu) {
+        const uint32_t c = da <= 1 ? 0u : 32u - (uint32_t)__clz(da - 1u);
+        b = (uint16_t)(min(c, (uint32_t)kBucketSide - 1u) * kBucketSide + vc);
+      }
+    }
+    ebk[e] = b;
+  }
+}
+
 // Keys of the transposed index: (target rank, edge id); sorted on the target
 // bits (stably, so edge ids stay ascending within a target).
 // Each edge's in-edge record word, (length of N+(u) after v) << 1 | sign,
@@ -632,7 +655,7 @@ __global__ void k_tkeys(const uint32_t *__restrict__ col, const uint32_t *__rest
 }
 
 // In-edge records of the sorted keys (edge e = (u -> v): the part of N+(u)
-// after v, the edge's sign and v's out-degree class), in-degrees, and each
+// after v and the edge's sign), in-degrees, and each
 // target's suffix-class
 // mask: bit c of smask[v] is set when some in-edge's part after v is
 // non-empty and of class min(ceil_log2 len, 15) -- the source side of the
@@ -643,8 +666,7 @@ __global__ void k_tkeys(const uint32_t *__restrict__ col, const uint32_t *__rest
 // touching lane 0 or lane 63 may continue in a neighbouring wave and is
 // added (or or-ed) atomically into the zeroed arrays.
 __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m, uint32_t n,
-                         const uint32_t *__restrict__ rec, const uint8_t *__restrict__ dcls,
-                         InEdge *__restrict__ tin,
+                         const uint32_t *__restrict__ rec, InEdge *__restrict__ tin,
                          uint32_t *__restrict__ tdeg, uint32_t *__restrict__ smask,
                          uint32_t *__restrict__ err) {
   const uint32_t lane = __lane_id();
@@ -664,8 +686,7 @@ __global__ void k_tsplit(const uint64_t *__restrict__ key, uint64_t m, uint32_t 
       }
       const uint32_t r = rec[e];
       const uint32_t len = r >> 1;
-      const uint32_t vc = v < n ? dcls[v] : 0xFFu;
-      tin[i] = InEdge{e + 1u, r | ((vc == 0xFFu ? kNoClass : vc) << 26)};
+      tin[i] = InEdge{e + 1u, r};
       const uint32_t c = len <= 1 ? 0u : 32u - (uint32_t)__clz(len - 1u);
       cbit = len ? 1u << min(c, 15u) : 0u;
     }
@@ -996,6 +1017,12 @@ hipError_t build_finish(const BuildInput &in, BuildBuffers &bb, BuildResult &res
   hipLaunchKernelGGL(k_dcls, dim3(grid_for((uint64_t)n + 1, T)), dim3(T), 0, st,
                      bb.off.as<uint32_t>(), n, bb.dcls.as<uint8_t>());
   STC_TRY(hipGetLastError());
+  const uint64_t mp = (m + 7) & ~7ull;
+  STC_TRY(bb.ebk.ensure((mp ? mp : 8) * 2));
+  hipLaunchKernelGGL(k_edge_buckets, dim3(grid_for(mp, T)), dim3(T), 0, st,
+                     bb.off.as<uint32_t>(), bb.col.as<uint32_t>(), bb.src.as<uint32_t>(),
+                     bb.dcls.as<uint8_t>(), m, mp, bb.ebk.as<uint16_t>());
+  STC_TRY(hipGetLastError());
   // transposed index for the pull strategy: in-edge records grouped by target rank
   STC_TRY(bb.tin.ensure((m ? m : 1) * sizeof(InEdge)));
   STC_TRY(bb.tin_off.ensure(((uint64_t)n + 1) * 4));
@@ -1011,8 +1038,7 @@ hipError_t build_finish(const BuildInput &in, BuildBuffers &bb, BuildResult &res
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n + 1, T)), dim3(T), 0, st,
                      bb.smask.as<uint32_t>(), (uint64_t)n + 1, 0u);
   hipLaunchKernelGGL(k_tsplit, dim3(grid_for(m, T)), dim3(T), 0, st, kA, m, n, rec,
-                     bb.dcls.as<uint8_t>(), bb.tin.as<InEdge>(), deg, bb.smask.as<uint32_t>(),
-                     id_err);
+                     bb.tin.as<InEdge>(), deg, bb.smask.as<uint32_t>(), id_err);
   STC_TRY(hipGetLastError());
   STC_TRY(scan_exclusive_u32(deg, bb.tin_off.as<uint32_t>(), (uint64_t)n + 1, bb.scr, st));
   // algorithmic bytes and max out-degree

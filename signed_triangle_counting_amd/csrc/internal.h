@@ -97,16 +97,15 @@ struct DispatchParams {
 
 // One in-edge e = (u -> v) of the transposed index: the part of N+(u) after
 // v is col[pos, pos + len) with pos = e + 1.  The word w holds the edge's sign
-// in bit 0, len in bits 1..25 and v's out-degree class min(ceil_log2 d+(v),
-// 15) in bits 26..30 (kNoClass when d+(v) = 0), so a record alone gives the
-// edge's binning bucket.  len <= d+(u) <= sqrt(2 m) < 2^17: every out-
-// neighbour of u has degree >= deg(u) >= d+(u) under the (degree, id) order.
+// in bit 0 and len in the bits above.
 struct InEdge {
   uint32_t pos, w;
 };
-constexpr uint32_t kNoClass = 31;
-__host__ __device__ inline uint32_t in_len(uint32_t w) { return (w >> 1) & ((1u << 25) - 1u); }
-__host__ __device__ inline uint32_t in_cls(uint32_t w) { return w >> 26; }
+__host__ __device__ inline uint32_t in_len(uint32_t w) { return w >> 1; }
+// Edge bucket of an oriented edge that can close no triangle (an empty part
+// of N+(u) after v, or an empty N+(v)); every other edge's bucket is
+// 16 min(ceil_log2 |N+(u) after v|, 15) + min(ceil_log2 d+(v), 15), 0..255.
+constexpr uint16_t kNoEdgeBucket = 0xFFFF;
 
 // Device view of the oriented CSR (rank space).
 struct DevCsr {
@@ -118,6 +117,9 @@ struct DevCsr {
   const uint32_t *tin_off = nullptr;  // n + 1: in-edges of each target rank
   const InEdge *tin = nullptr;        // m: in-edges grouped by target, ascending e
   const uint8_t *dcls = nullptr;      // n: min(ceil_log2 d+(v), 15), 0xFF when d+(v) = 0
+  // m rounded up to 8: each oriented edge's binning bucket, in edge order
+  // (kNoEdgeBucket past m)
+  const uint16_t *ebk = nullptr;
   // n: bit c set when some in-edge (u -> v) has a non-empty part of N+(u)
   // after v, of length len with min(ceil_log2 len, 15) = c
   const uint32_t *smask = nullptr;
@@ -219,7 +221,7 @@ struct BuildBuffers {
   DevBuf in_u, in_v, in_s;
   DevBuf keyA, keyB, flag, pos;
   DevBuf deg, vkeyA, vkeyB, rank, perm, graph_of;
-  DevBuf col, src, off, fdeg, tin, tin_off, dcls, smask;
+  DevBuf col, src, off, fdeg, tin, tin_off, dcls, smask, ebk;
   DevBuf misc, sort;
   ScanScratch scr;
 };

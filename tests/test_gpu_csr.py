@@ -64,8 +64,7 @@ def test_device_csr_equals_oracle_forward_csr(ctx, case):
     want_col = (dst_r[order] << 1) | (S[order] < 0)
     assert (a["col"].astype(np.int64) == want_col).all()
     # transposed index: per target rank, its in-edges e ascending, as
-    # (e + 1, (class of d+(v) << 26) | (|N+(u) after v| << 1) | neg), the
-    # class min(ceil_log2 d+(v), 15), or 31 when d+(v) = 0
+    # (e + 1, (|N+(u) after v| << 1) | neg)
     src_e, dst_e = src_r[order], dst_r[order]
     e = np.arange(m, dtype=np.int64)
     suffix = want_off[src_e + 1] - e - 1
@@ -73,9 +72,7 @@ def test_device_csr_equals_oracle_forward_csr(ctx, case):
     want_tin_off = np.concatenate([[0], np.cumsum(np.bincount(dst_e, minlength=n))])
     assert (a["tin_off"].astype(np.int64) == want_tin_off).all()
     want_dcls = np.where(fdr == 0, 0xFF, np.minimum(clog2(fdr), 15))
-    tcls = np.where(fdr[dst_e] == 0, 31, np.minimum(clog2(fdr[dst_e]), 15))
-    want_tin = np.stack([e[by_t] + 1, (tcls[by_t] << 26) | (suffix[by_t] << 1) | (S[order][by_t] < 0)],
-                        axis=1)
+    want_tin = np.stack([e[by_t] + 1, (suffix[by_t] << 1) | (S[order][by_t] < 0)], axis=1)
     assert (a["tin"].astype(np.int64) == want_tin).all()
     # per-rank out-degree class and suffix-class mask (binning's two halves)
     assert (a["dcls"].astype(np.int64) == want_dcls).all()
@@ -83,6 +80,13 @@ def test_device_csr_equals_oracle_forward_csr(ctx, case):
     want_smask = np.zeros(n, np.int64)
     np.bitwise_or.at(want_smask, dst_e, bits)
     assert (a["smask"].astype(np.int64) == want_smask).all()
+    # the bucket histogram of the binning pass: per oriented edge with a
+    # non-empty suffix and a non-empty N+(v), bucket 16 min(ceil_log2 |suffix|,
+    # 15) + min(ceil_log2 d+(v), 15)
+    live = (suffix > 0) & (fdr[dst_e] > 0)
+    want_bk = np.minimum(clog2(suffix[live]), 15) * 16 + np.minimum(clog2(fdr[dst_e][live]), 15)
+    want_hist = np.bincount(want_bk, minlength=256)
+    assert (ctx.bucket_histogram().astype(np.int64) == want_hist).all()
     # the merge model's bytes, 4 * sum over oriented edges of (d+(u) + d+(v))
     # + 4 per edge + 8 per vertex, and the largest out-degree
     words = int((fdr[src_r] + fdr[dst_r]).sum())
