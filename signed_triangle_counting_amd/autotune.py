@@ -18,6 +18,9 @@ the benchmark workloads:
   pull      pull_block_chunk_log2 / pull_wave_chunk_log2: in-edges per
             workgroup / wave pull item; wave_pop: items a wave takes per queue
             pop;
+  wavepull  pull_wave_max_log2 / batch_pull_wave_max_log2: the longest
+            target list (over a span within the wave window) counted by a
+            wave instead of a workgroup, in single graphs / in batches;
   batch     lane_item_log2: merge-lane edges per workgroup item of the batched
             kernel (timed on small-graph batches);
   fixed     the fixed rule's four thresholds (fixed_merge_log2,
@@ -46,12 +49,14 @@ import signed_triangle_counting_amd as stc  # noqa: E402
 
 KEYS = ["merge_wave_min_log2", "hash_max_log2", "bitmap_span_log2", "wave_window_log2",
         "fixed_merge_log2", "fixed_pull_log2", "fixed_skew_log2", "fixed_hub_log2",
-        "pull_block_chunk_log2", "pull_wave_chunk_log2", "wave_pop", "lane_item_log2"]
+        "pull_block_chunk_log2", "pull_wave_chunk_log2", "pull_wave_max_log2",
+        "batch_pull_wave_max_log2", "wave_pop", "lane_item_log2"]
 
 DEFAULT = {"merge_wave_min_log2": [5] * 16, "hash_max_log2": 10, "bitmap_span_log2": 17,
            "wave_window_log2": 14, "fixed_merge_log2": 3, "fixed_pull_log2": 0,
            "fixed_skew_log2": 2, "fixed_hub_log2": 9, "pull_block_chunk_log2": 11,
-           "pull_wave_chunk_log2": 8, "wave_pop": 8, "lane_item_log2": 9}
+           "pull_wave_chunk_log2": 8, "pull_wave_max_log2": 6, "batch_pull_wave_max_log2": 6,
+           "wave_pop": 8, "lane_item_log2": 9}
 
 # Values each parameter is swept over (merge_wave_min_log2: as one global value).
 CANDIDATES = {
@@ -61,6 +66,8 @@ CANDIDATES = {
     "hash_max_log2": [9, 10, 11],
     "pull_block_chunk_log2": [9, 10, 11, 12, 13],
     "pull_wave_chunk_log2": [5, 6, 7, 8, 9, 10],
+    "pull_wave_max_log2": [6, 7, 8, 9],
+    "batch_pull_wave_max_log2": [6, 7, 8, 9],
     "wave_pop": [2, 4, 8],
     "lane_item_log2": [9, 10, 11],
     # the fixed rule: 16 switches the pull, skew and hub branches off, -1 the
@@ -226,7 +233,7 @@ def tune_fixed(ctx, timer, best, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--only", default="merge,bitmap,hash,pull,batch,fixed")
+    ap.add_argument("--only", default="merge,bitmap,hash,pull,wavepull,batch,fixed")
     ap.add_argument("--start", default=None, help="tune.txt to start from (default: defaults)")
     ap.add_argument("--reps", type=int, default=3)
     args = ap.parse_args()
@@ -283,7 +290,7 @@ def main():
         res = sweep(timer, items(), best, "hash_max_log2", "hash", log)
         best["hash_max_log2"] = min(res, key=res.get)
     batches = None
-    if "pull" in sections or "batch" in sections:
+    if "pull" in sections or "wavepull" in sections or "batch" in sections:
         batches = [(name, thunk()) for name, thunk in tuning_batches()]
     bitems = [(name, (lambda b=b: ctx.prepare_batch(b))) for name, b in (batches or [])]
     if "pull" in sections:
@@ -293,6 +300,18 @@ def main():
             resb = sweep(timer, bitems, best, key, "pull", log, batch=True)
             tot = {v: res[v] + resb[v] for v in res}
             best[key] = min(tot, key=tot.get)
+    if "wavepull" in sections:
+        # a candidate replaces the starting value only when at least 1% faster
+        # (about the run-to-run spread of these sums)
+        def pick(res, cur):
+            b = min(res, key=res.get)
+            return b if res[b] < 0.99 * res.get(cur, float("inf")) else cur
+        log("pull_wave_max_log2 (single graphs)")
+        res = sweep(timer, items(), best, "pull_wave_max_log2", "pull", log)
+        best["pull_wave_max_log2"] = pick(res, best["pull_wave_max_log2"])
+        log("batch_pull_wave_max_log2 (batches)")
+        res = sweep(timer, bitems, best, "batch_pull_wave_max_log2", "pull", log, batch=True)
+        best["batch_pull_wave_max_log2"] = pick(res, best["batch_pull_wave_max_log2"])
     if "batch" in sections:
         log("batched merge-lane item")
         res = sweep(timer, bitems, best, "lane_item_log2", "merge", log, batch=True)

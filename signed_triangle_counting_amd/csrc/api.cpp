@@ -121,6 +121,8 @@ void default_tune(TuneParams &t) {
   t.fixed_hub_log2 = 9;
   t.pull_block_chunk_log2 = 11;
   t.pull_wave_chunk_log2 = 8;
+  t.pull_wave_max_log2 = 6;
+  t.batch_pull_wave_max_log2 = 6;
   t.wave_pop = 8;
   t.lane_item_log2 = 9;
 }
@@ -154,10 +156,11 @@ bool parse_tune(std::istream &f, TuneParams &t) {
     std::string k;
     if (!(is >> k) || k[0] == '#') continue;
     if (k == "merge_wave_min_log2") {
-      for (int i = 0; i < kBucketSide; i++)
-        if (!(is >> x.merge_wave_min_log2[i]) || x.merge_wave_min_log2[i] < 0 ||
-            x.merge_wave_min_log2[i] > 16)
-          return false;
+      for (int i = 0; i < kBucketSide; i++) {
+        int v = 0;
+        if (!(is >> v) || v < 0 || v > 16) return false;
+        x.merge_wave_min_log2[i] = (int8_t)v;
+      }
     } else if (k == "hash_max_log2") {
       if (!(is >> x.hash_max_log2) || x.hash_max_log2 < 4 || x.hash_max_log2 > 12) return false;
     } else if (k == "bitmap_span_log2") {
@@ -185,6 +188,14 @@ bool parse_tune(std::istream &f, TuneParams &t) {
     } else if (k == "pull_wave_chunk_log2") {
       if (!(is >> x.pull_wave_chunk_log2) || x.pull_wave_chunk_log2 < 5 ||
           x.pull_wave_chunk_log2 > 12)
+        return false;
+    } else if (k == "pull_wave_max_log2") {
+      if (!(is >> x.pull_wave_max_log2) || x.pull_wave_max_log2 < 6 ||
+          x.pull_wave_max_log2 > 9)
+        return false;
+    } else if (k == "batch_pull_wave_max_log2") {
+      if (!(is >> x.batch_pull_wave_max_log2) || x.batch_pull_wave_max_log2 < 6 ||
+          x.batch_pull_wave_max_log2 > 9)
         return false;
     } else if (k == "wave_pop") {
       if (!(is >> x.wave_pop) || x.wave_pop < 1 || x.wave_pop > 8) return false;
@@ -506,8 +517,10 @@ int stc_prepare(stc_ctx *c, const stc_edges *g, int g_is_device) {
 // Dispatch parameters to the device for the next count: the host's copy,
 // plus whether every class-11 target list of the prepared graph fits the
 // current bitmap window (binning then needs no per-edge pass for pull).
-static hipError_t upload_params(stc_ctx *c) {
+static hipError_t upload_params(stc_ctx *c, bool batch) {
   c->hp.pull_c11_fits = c->br.max_span_c11 <= (1u << c->hp.tune.bitmap_span_log2) ? 1u : 0u;
+  c->hp.wave_pull_max_log2 =
+      (uint32_t)(batch ? c->hp.tune.batch_pull_wave_max_log2 : c->hp.tune.pull_wave_max_log2);
   return hipMemcpyAsync(c->dp, &c->hp, sizeof(DispatchParams), hipMemcpyHostToDevice, c->st);
 }
 
@@ -523,7 +536,7 @@ static int count_core(stc_ctx *c, int mode, const uint8_t *table, int repeat, bo
   c->hp.tune.allow_pull = 1;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   graph_features(c->g.n, c->g.m, 1, c->hp.graph_feat);
-  hipError_t e = upload_params(c);
+  hipError_t e = upload_params(c, false);
   if (e != hipSuccess) return hip_fail(c, e, "params upload");
   // The event marks between the count's kernels (binning and per-kernel
   // times) are recorded in the first half of the runs only; with repeats,
@@ -837,7 +850,7 @@ static int batch_table_core(stc_ctx *c, int mode, const uint8_t *table, int repe
   c->hp.tune.allow_pull = 1;
   if (table) std::memcpy(c->hp.host_table, table, kBuckets);
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
-  hipError_t e = upload_params(c);
+  hipError_t e = upload_params(c, true);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
   const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
   std::vector<float> times, bins, kt[kKernels];
@@ -929,7 +942,7 @@ extern "C" int stc_count_batch_prepared(stc_ctx *c, const stc_opts *opts, stc_co
   c->hp.mode = dispatch;
   c->hp.tune.allow_pull = 1;
   graph_features(c->g.n, c->g.m, k, c->hp.graph_feat);
-  hipError_t e = upload_params(c);
+  hipError_t e = upload_params(c, true);
   unsigned long long *d_pg = c->batch_counts.as<unsigned long long>();
   const uint32_t *graph_of = c->bb.graph_of.as<uint32_t>();
   // eager run (allocates every buffer; its counts are what each replay must

@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "internal.h"
 
@@ -529,20 +531,29 @@ __device__ __forceinline__ bool any_pull_in(const uint16_t *pull_src) {
 // target is a pull vertex when some in-edge lies in a bucket binned to pull
 // (its suffix-class mask meets pull_src of its own class) and its list fits
 // the pull kernels -- exactly when some in-edge's strategy is pull, since
-// the fit test reads the target alone.
+// the fit test reads the target alone.  A pull target's chunks are wave
+// items when its list has at most kWaveMaxDeg entries, or at most
+// 2^wave_max_log2 entries over a rank span within the wave window (no entry
+// then falls below the window); else workgroup items.
 __device__ __forceinline__ void vertex_lists_of(uint32_t v, const DevCsr &g, const uint32_t *vflag,
                                                 const uint16_t *pull_src, bool any_pull,
-                                                const TuneParams &t, uint32_t c[kVLists],
-                                                uint32_t &f) {
+                                                const TuneParams &t, uint32_t wave_max_log2,
+                                                uint32_t c[kVLists], uint32_t &f) {
   f = 0;
-  bool small = false, pull = false;
+  bool small = false, small_pull = false, pull = false;
   uint32_t indeg = 0;
   if (v < g.n) {
     f = vflag[v];
     const uint32_t cb = any_pull ? (uint32_t)g.dcls[v] : 0xFFu;  // no pull bucket: no reads
     if (cb != 0xFFu && (g.smask[v] & pull_src[cb]))
       pull = pull_fits(t, (int)cb, [&] { return span_of(g.col, g.off[v], g.off[v + 1]); });
-    if (f || pull) small = g.off[v + 1] - g.off[v] <= kWaveMaxDeg;
+    if (f || pull) {
+      const uint32_t d = g.off[v + 1] - g.off[v];
+      small = d <= kWaveMaxDeg;
+      small_pull = small;
+      if (pull && !small && d <= (1u << wave_max_log2))
+        small_pull = span_of(g.col, g.off[v], g.off[v + 1]) <= (1u << t.wave_window_log2);
+    }
     if (pull) indeg = g.tin_off[v + 1] - g.tin_off[v];
   }
   c[0] = (f & 1u) && !small;
@@ -550,8 +561,8 @@ __device__ __forceinline__ void vertex_lists_of(uint32_t v, const DevCsr &g, con
   c[2] = (f & 1u) && small;
   c[3] = (f & 2u) && small;
   const uint32_t cb_ = 1u << t.pull_block_chunk_log2, cw = 1u << t.pull_wave_chunk_log2;
-  c[4] = pull && !small ? (indeg + cb_ - 1) / cb_ : 0u;
-  c[5] = pull && small ? (indeg + cw - 1) / cw : 0u;
+  c[4] = pull && !small_pull ? (indeg + cb_ - 1) / cb_ : 0u;
+  c[5] = pull && small_pull ? (indeg + cw - 1) / cw : 0u;
 }
 
 // Inclusive sum over the wave (every lane active) by DPP: row_shr 1, 2, 4, 8
@@ -601,13 +612,14 @@ __global__ __launch_bounds__(256) void k_vcount(DevCsr g, const Ctrl *ctrl,
     return;
   }
   const bool any_pull = any_pull_in(s_pull);
+  const uint32_t wmax = P->wave_pull_max_log2;
   uint32_t cnt[kVLists] = {0, 0, 0, 0, 0, 0};
   const uint32_t base = blockIdx.x * kVTile;
 #pragma unroll 4  // rounds are independent: their loads overlap
   for (uint32_t r = 0; r < kVTile; r += 256) {
     uint32_t c[kVLists];
     uint32_t f;
-    vertex_lists_of(base + r + tid, g, vflag, s_pull, any_pull, tune, c, f);
+    vertex_lists_of(base + r + tid, g, vflag, s_pull, any_pull, tune, wmax, c, f);
 #pragma unroll
     for (int L = 0; L < kVLists; L++) cnt[L] += c[L];
   }
@@ -653,18 +665,20 @@ __global__ __launch_bounds__(256) void k_vscatter(DevCsr g, const DispatchParams
   pull_sources_of(ctrl, s_pull);
   __syncthreads();
   const bool any_pull = any_pull_in(s_pull);
+  const uint32_t wmax = P->wave_pull_max_log2;
   const uint32_t base = b * kVTile;
   // the next round's lists are found while this round's are written (their
   // loads are in flight across this round's barriers)
   uint32_t cn[kVLists], fn;
-  vertex_lists_of(base + tid, g, vflag, s_pull, any_pull, tune, cn, fn);
+  vertex_lists_of(base + tid, g, vflag, s_pull, any_pull, tune, wmax, cn, fn);
   for (uint32_t r = 0; r < kVTile; r += 256) {
     const uint32_t v = base + r + tid;
     uint32_t c[kVLists];
 #pragma unroll
     for (int L = 0; L < kVLists; L++) c[L] = cn[L];
     const uint32_t f = fn;
-    if (r + 256 < kVTile) vertex_lists_of(v + 256, g, vflag, s_pull, any_pull, tune, cn, fn);
+    if (r + 256 < kVTile)
+      vertex_lists_of(v + 256, g, vflag, s_pull, any_pull, tune, wmax, cn, fn);
     uint32_t incl[kVLists];
 #pragma unroll
     for (int L = 0; L < kVLists; L++) {
@@ -1766,14 +1780,29 @@ __device__ __forceinline__ void unstage_block(const DevCsr &g, const Staged &st,
   __syncthreads();
 }
 
-// Wave staging of a list of at most kWaveMaxDeg entries in the wave's own B
-// (zero on entry) and L (kWaveMaxDeg words); the lane's entry is kept for
-// unstage_wave.
+// Wave staging of a list in the wave's own B (zero on entry) and L
+// (kWaveMaxDeg words): lane k stages entry k, and the lane's entry is kept
+// for unstage_wave.  A list of more than kWaveMaxDeg entries is wave-staged
+// only when it spans at most the window (a pull target's list, so chosen by
+// the vertex-list pass): its window starts at its first entry, no entry is
+// low, and stage_wave_rest sets the bits of entries 64 and up.
 struct StagedWave {
   Staged st;
   bool in_w;
   uint32_t o;
 };
+
+// Entries [k0, k1) of a wave-staged list longer than a wave (every one in
+// the window; the test only keeps a list that breaks that rule inside B).
+__device__ __forceinline__ void stage_wave_rest(const uint32_t *__restrict__ col, uint32_t k0,
+                                                uint32_t k1, uint32_t base, uint32_t window,
+                                                uint32_t *B) {
+  for (uint32_t k = k0 + __lane_id(); k < k1; k += 64u) {
+    const uint32_t x = col[k];
+    const uint32_t o = (x >> 1) - base;
+    if (o < window) atomicOr(&B[o >> 4], 1u << ((o & 15u) * 2u + (x & 1u)));
+  }
+}
 
 // The list col[la, ea) (offsets known to the caller).
 __device__ __forceinline__ StagedWave stage_wave_at(const DevCsr &g, uint32_t la, uint32_t ea,
@@ -1799,6 +1828,7 @@ __device__ __forceinline__ StagedWave stage_wave_at(const DevCsr &g, uint32_t la
   sw.in_w = mine && (w >> 1) >= st.base;
   if (sw.in_w) atomicOr(&B[sw.o >> 4], 1u << ((sw.o & 15u) * 2u + (w & 1u)));
   else if (mine) L[lane] = w;  // the low entries are the lanes [0, nlow)
+  if (st.d > kWaveMaxDeg) stage_wave_rest(col, la + 64u, ea, st.base, window, B);
   st.nlow = (uint32_t)__popcll(__ballot(mine && !sw.in_w));
   wave_lds_sync();
   return sw;
@@ -1812,7 +1842,13 @@ __device__ __forceinline__ StagedWave stage_wave(const DevCsr &g, uint32_t x, ui
 
 __device__ __forceinline__ void unstage_wave(const StagedWave &sw, uint32_t *B) {
   wave_lds_sync();
-  if (sw.in_w) B[sw.o >> 4] = 0u;
+  if (sw.st.d > kWaveMaxDeg) {
+    // a longer list: the words of its whole span (at most the window)
+    const uint32_t words = (sw.st.span + 15u) >> 4;
+    for (uint32_t k = __lane_id(); k < words; k += 64u) B[k] = 0u;
+  } else if (sw.in_w) {
+    B[sw.o >> 4] = 0u;
+  }
   wave_lds_sync();
 }
 
@@ -1920,8 +1956,9 @@ __device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, 
 }
 
 // One wave: the in-edges [j0, t1) of a target whose list is col[la, ea) (at
-// most kWaveMaxDeg entries).  The list's entries (one per lane, whose first
-// and last give the rank window), the first 64 in-edge records and the
+// most kWaveMaxDeg entries, or more over a span within the window).  The
+// list's first 64 entries (one per lane; the first and last entries give the
+// rank window), the first 64 in-edge records and the
 // target class's pull mask are fetched together, so an item starts with one
 // memory round trip instead of a chain of four.  The mask has bit c set when
 // bucket (c, class of d+(v)) is binned to pull: an in-edge is pull iff its
@@ -1943,8 +1980,10 @@ __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, u
   const uint32_t *__restrict__ col = g.col;
   const uint32_t lane = __lane_id();
   const uint32_t d = ea - la;
+  // lane 63 of a longer list takes its last entry (the top of the window),
+  // and entries 63 .. d - 2 are staged after the first load
   uint32_t w = 0;
-  if (lane < d) w = col[la + lane];
+  if (lane < d) w = col[lane < 63u || d <= 64u ? la + lane : ea - 1u];
   InEdge r0{0u, 0u};
   if (j0 + lane < t1) r0 = g.tin[j0 + lane];
   const int cb = min(clog2(d), kBucketSide - 1);
@@ -1957,7 +1996,7 @@ __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, u
   st.ea = ea;
   st.d = d;
   st.lo = (uint32_t)__builtin_amdgcn_readlane((int)w, 0) >> 1;
-  st.hi = (uint32_t)__builtin_amdgcn_readlane((int)w, (int)d - 1) >> 1;
+  st.hi = (uint32_t)__builtin_amdgcn_readlane((int)w, (int)min(d, 64u) - 1) >> 1;
   st.span = st.hi - st.lo + 1u;
   st.base = window_base(st.lo, st.hi, window);
   const bool mine = lane < d;
@@ -1965,6 +2004,7 @@ __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, u
   sw.in_w = mine && (w >> 1) >= st.base;
   if (sw.in_w) atomicOr(&B[sw.o >> 4], 1u << ((sw.o & 15u) * 2u + (w & 1u)));
   else if (mine) L[lane] = w;  // the low entries are the lanes [0, nlow)
+  if (d > kWaveMaxDeg) stage_wave_rest(col, la + 63u, ea - 1u, st.base, window, B);
   st.nlow = (uint32_t)__popcll(__ballot(mine && !sw.in_w));
   wave_lds_sync();
   // the first metadata batch's records are the ones fetched above
@@ -1978,9 +2018,15 @@ __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, u
   const auto keep = [&](uint32_t, uint32_t pb, uint32_t eb) {
     return eb > pb && ((pmask >> min(clog2(eb - pb), kBucketSide - 1)) & 1u);
   };
-  st.with_probe(B, L, [&](const auto &probe) {
-    stream_packed<CLS>(g, j0, t1, 1u, meta, keep, probe, H, acc);
-  });
+  // a longer list's in-edges carry longer suffixes: rows suit them better
+  if (d > kWaveMaxDeg)
+    st.with_probe(B, L, [&](const auto &probe) {
+      stream_rows<CLS>(g, j0, t1, 1u, meta, keep, probe, acc);
+    });
+  else
+    st.with_probe(B, L, [&](const auto &probe) {
+      stream_packed<CLS>(g, j0, t1, 1u, meta, keep, probe, H, acc);
+    });
   unstage_wave(sw, B);
 }
 
@@ -2005,7 +2051,7 @@ template <bool CLS>
 __global__ __launch_bounds__(kVertexBlock) void k_hash(DevCsr g, Ctrl *ctrl,
                                                        const DispatchParams *P,
                                                        const uint32_t *__restrict__ vlist) {
-  extern __shared__ uint32_t T[];
+  extern __shared__ __attribute__((aligned(128))) uint32_t T[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count;
   __shared__ TuneParams tune;
@@ -2102,7 +2148,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_bitmap(DevCsr g, Ctrl *ctrl,
                                                          const DispatchParams *P,
                                                          const uint32_t *__restrict__ vlist,
                                                          uint32_t bitmap_words) {
-  extern __shared__ uint32_t B[];
+  extern __shared__ __attribute__((aligned(128))) uint32_t B[];
   uint32_t *L = B + bitmap_words + kGuard;
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
@@ -2130,7 +2176,7 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
                                                      const uint32_t *__restrict__ vlist,
                                                      const uint32_t *__restrict__ vtarget,
                                                      uint32_t window_words, uint32_t defer) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __attribute__((aligned(128))) uint32_t lds[];
   __shared__ uint8_t table[kBuckets];
   __shared__ TuneParams tune;
   __shared__ uint64_t s_heads[PULL ? 4 * kPackWords / 2 : 1];  // packed streaming, per wave
@@ -2254,7 +2300,10 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
   // four-class kernel would have to count merge-lane items in four classes
   static_assert(!CLS || KIND == kKindPullOne, "four-class counting is one graph's pull items");
   constexpr bool ALL = KIND == kKindAll;
-  extern __shared__ uint32_t R[];
+  // the region starts on a 128-byte boundary whatever the static arrays take:
+  // the wave records are 8-byte words, and the start that followed the static
+  // arrays (4-byte aligned) once left them misaligned, 13% slower on a batch
+  extern __shared__ __attribute__((aligned(128))) uint32_t R[];
   __shared__ uint8_t table[kBuckets];
   __shared__ uint32_t s_first, s_count, s_nlow;
   __shared__ TuneParams tune;
@@ -2702,6 +2751,30 @@ hipError_t enqueue_pv_to_ids(const unsigned long long *pv_rank, const uint32_t *
   return hipGetLastError();
 }
 
+// Workgroups of `threads` threads and `lds` dynamic LDS bytes that one CU
+// holds at once, cached per kernel and size (the runtime query is made once,
+// before any stream capture of the batch, whose eager run comes first).
+hipError_t resident_blocks(int *out, const void *kern, int threads, size_t lds) {
+  struct Entry {
+    const void *kern;
+    size_t lds;
+    int blocks;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const Entry &e : cache)
+    if (e.kern == kern && e.lds == lds) {
+      *out = e.blocks;
+      return hipSuccess;
+    }
+  int b = 0;
+  STC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, threads, lds));
+  cache.push_back(Entry{kern, lds, b});
+  *out = b;
+  return hipSuccess;
+}
+
 // Words of the batch kernel's shared LDS region (the largest structure).
 static size_t batch_region_words(const TuneParams &t) {
   const size_t bitmap = (size_t)((1u << t.bitmap_span_log2) / 16u) + kGuard + kLowCap;
@@ -2722,9 +2795,15 @@ hipError_t launch_two_phase(const DevCsr &g, const DispatchParams &hp, const Dis
   // four-class counting is one graph's (its pull items); batches are two-class
   if (classes && !one_graph) return hipErrorInvalidValue;
   const size_t lds = batch_lds_bytes(hp.tune);
-  // 160 KB of LDS per CU: the region plus about 0.4 KB of static LDS per workgroup
-  const int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (156 * 1024) / lds));
   auto launch = [&](auto kern) -> hipError_t {
+    // The persistent grid is as many workgroups as fit on the CUs together
+    // (at most 4 per CU): the runtime's count, which rounds each
+    // workgroup's LDS (the region plus the static arrays) to the hardware's
+    // allocation unit.  A count from the byte sizes alone once launched a
+    // fourth workgroup per CU that waited for a third to finish.
+    int blocks_per_cu = 0;
+    STC_TRY(resident_blocks(&blocks_per_cu, (const void *)kern, kBatchBlock, lds));
+    blocks_per_cu = std::max(1, std::min(4, blocks_per_cu));
     hipLaunchKernelGGL(kern, dim3(num_cus * blocks_per_cu), dim3(kBatchBlock), lds, st, g,
                        cb.gcount.as<Ctrl>(), dp, L.lane_list, L.wave_list, L.vh, L.vstride,
                        graph_of, per_graph, (1u << hp.tune.bitmap_span_log2) / 16u,

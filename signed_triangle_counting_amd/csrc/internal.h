@@ -61,7 +61,7 @@ enum DispatchMode : int {
 struct TuneParams {
   // merge lane class: buckets whose longer side is <= 2^wave_min_log2 use one
   // lane per edge, else one wave per edge
-  int merge_wave_min_log2[kBucketSide];  // indexed by log2 of shorter side
+  int8_t merge_wave_min_log2[kBucketSide];  // indexed by log2 of shorter side
   int hash_max_log2;      // hash only if the source list has <= 2^x entries
   int bitmap_span_log2;   // rank window of the workgroup bitmap (entries)
   int wave_window_log2;   // rank window of the per-wave bitmap (entries)
@@ -75,6 +75,11 @@ struct TuneParams {
   // Item sizes of the vertex-grouped and batched kernels:
   int pull_block_chunk_log2;  // in-edges per workgroup-pull item (8..14)
   int pull_wave_chunk_log2;   // in-edges per wave-pull item (5..12)
+  // pull targets with 64 < d+ <= 2^x entries whose list spans at most the
+  // wave window are counted by waves too (6..9; 6: none), in single graphs
+  // and in batches of small graphs
+  int pull_wave_max_log2;
+  int batch_pull_wave_max_log2;
   int wave_pop;               // items a wave takes per queue pop (1..8)
   int lane_item_log2;         // merge-lane edges per batched workgroup item (9..12)
   int allow_pull;         // 1: pull buckets are counted by the pull kernels (single
@@ -93,6 +98,8 @@ struct DispatchParams {
   // at most the workgroup bitmap window, so a pull bucket of class <= 11 never
   // demotes an edge (from the prepared graph's largest such span)
   uint32_t pull_c11_fits;
+  // the wave-pull limit of this count (pull_wave_max_log2, or its batch value)
+  uint32_t wave_pull_max_log2;
 };
 
 // One in-edge e = (u -> v) of the transposed index: the part of N+(u) after

@@ -62,3 +62,26 @@ def test_every_tune_candidate_counts_batch_exactly(ctx):
                 assert ctx.count_batch_prepared(dispatch=mode) == want, (key, v, mode)
     finally:
         assert ctx.load_models() == 0
+
+
+def test_long_wave_pull_lists_four_classes_and_per_vertex(ctx):
+    """Pull targets of up to 512 entries counted by waves (pull_wave_max_log2
+    9: the list staged 64 entries at a time, its in-edges streamed as rows):
+    two-class, four-class and per-vertex counts against the oracle."""
+    g = stc.gen_rmat(16, 16, 0.57, 0.19, 0.19, 0.3, 8083)
+    o = oracle.Csr(g.n, g.u, g.v, g.sign)
+    want2, want4, want_pv = o.merge(), o.classes(), o.classes_per_vertex()
+    ctx.prepare(g)
+    p = {k: (list(x) if isinstance(x, list) else x) for k, x in at.DEFAULT.items()}
+    p["pull_wave_max_log2"] = 9
+    try:
+        ctx.set_tune(at.tune_text(p))
+        for mode in ("learned", "pull"):
+            say("long wave pull", mode)
+            assert ctx.count_prepared(dispatch=mode) == want2, mode
+            ctx.count_prepared(dispatch=mode, classes=True)
+            assert ctx.classes() == want4, mode
+            got = ctx.count_per_vertex(dispatch=mode)
+            assert (got == want_pv).all(), mode
+    finally:
+        assert ctx.load_models() == 0
