@@ -48,8 +48,14 @@ constexpr int kVertexBlock = 256;   // threads per workgroup of the hash kernel
 constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kernel
 
 // Device control block, cleared at the start of every counting run.
+// Copies of the bucket histogram that k_bin_hist's workgroups add into (by
+// blockIdx mod kHistSlots), so that no counter takes more than an eighth of
+// the workgroups' atomics; k_bin_table sums them into bucket_cnt.
+constexpr int kHistSlots = 8;
+
 struct Ctrl {
   uint32_t bucket_cnt[kBuckets];
+  uint32_t bucket_part[kHistSlots][kBuckets];
   uint32_t len[8];   // merge-lane list, merge-wave list, then vertex lists:
                      // workgroup hash, workgroup bitmap, wave hash, wave bitmap,
                      // workgroup pull, wave pull
@@ -351,7 +357,7 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
   uint32_t sum = 0;
 #pragma unroll
   for (uint32_t c = 0; c < kHistCopies; c++) sum += h[tid * kHistCopies + c];
-  if (sum) atomicAdd(&ctrl->bucket_cnt[tid], sum);
+  if (sum) atomicAdd(&ctrl->bucket_part[blockIdx.x % kHistSlots][tid], sum);
 }
 
 // The strategy table from the bucket histogram (one workgroup, thread =
@@ -360,7 +366,10 @@ __global__ __launch_bounds__(256, 8) void k_bin_hist(DevCsr g, Ctrl *ctrl) {
 // a device-scope fence.
 __global__ __launch_bounds__(kBuckets) void k_bin_table(Ctrl *ctrl, const DispatchParams *P) {
   const int tid = threadIdx.x;
-  const uint32_t cnt = ctrl->bucket_cnt[tid];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int s = 0; s < kHistSlots; s++) cnt += ctrl->bucket_part[s][tid];
+  ctrl->bucket_cnt[tid] = cnt;
   ctrl->table[tid] = cnt ? decide(tid, cnt, *P) : (uint8_t)S_SKIP;
 }
 
