@@ -52,6 +52,11 @@ constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kerne
 // blockIdx mod kHistSlots), so that no counter takes more than an eighth of
 // the workgroups' atomics; k_bin_table sums them into bucket_cnt.
 constexpr int kHistSlots = 8;
+// Copies of the class counters, one 128-byte line each: a wave adds its
+// totals into copy blockIdx mod kTSlots (thousands of waves end together, and
+// one counter pair took all of their atomics); the host sums the copies.
+constexpr int kTSlots = 8;
+constexpr int kTStride = 16;
 
 struct Ctrl {
   uint32_t bucket_cnt[kBuckets];
@@ -68,7 +73,7 @@ struct Ctrl {
                                      // graph's two-phase pull launch uses the
                                      // last two): one counter per XCD, 128
                                      // bytes apart
-  unsigned long long t[4];           // class counters
+  unsigned long long t[kTSlots][kTStride];  // class counters: copies, entries 0..3 each
   unsigned long long kern_edges[10]; // oriented edges counted per kernel
                                      // (launch order); slot 8: left out by
                                      // a partial table, slot 9: skipped
@@ -884,7 +889,7 @@ struct Acc {
   }
   __device__ __forceinline__ void flush(Ctrl *ctrl) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) wave_add(&ctrl->t[k], c[k]);
+    for (int k = 0; k < 4; k++) wave_add(&ctrl->t[blockIdx.x % kTSlots][k], c[k]);
   }
   // Batched counting: add this wave's totals to graph gid's (balanced,
   // unbalanced) pair and restart from zero.
@@ -1067,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_merge_lane(DevCsr g, Ctrl *ctrl,
   for (int k = 0; k < 4; k++) {
     uint64_t v = c[k];
     for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
-    wave_add(&ctrl->t[k], v);
+    wave_add(&ctrl->t[blockIdx.x % kTSlots][k], v);
   }
 }
 
@@ -2240,7 +2245,8 @@ __global__ __launch_bounds__(256) void k_bitmap_wave(DevCsr g, Ctrl *ctrl,
 //                   table's choice for batches of small graphs);
 //   kKindPullOne    one graph's pull items, as a batch of one: graph_of null,
 //                   queues q1, q2 that no other kernel of that count uses;
-//                   two-class totals go to per_graph[0..1], the four-class
+//                   two-class totals go to per_graph[0..1] of the class
+//                   counters' copy the workgroup picks, the four-class
 //                   variant (CLS) adds its classes to ctrl->t.  The two pull
 //                   phases then share one launch, with no boundary and tail
 //                   between the workgroup and the wave chunks.
@@ -2339,6 +2345,9 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
   if (tid == 0) tune = P->tune;
   for (uint32_t k = tid; k < region_words; k += blockDim.x) R[k] = 0u;
   __syncthreads();  // table, tune and the zeroed region are visible to every wave
+  // one graph (no graph_of): its counter pair is one copy of the class
+  // counters, chosen by workgroup
+  if (!graph_of) per_graph += (blockIdx.x % kTSlots) * kTStride;
   GraphAcc<CLS> ga;
   Acc<CLS> &acc = ga.acc;
   // phase 1: workgroup items [lane chunks | hash | bitmap | pull chunks]
@@ -2850,7 +2859,10 @@ hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_
   Ctrl h;
   STC_TRY(hipMemcpyAsync(&h, cb.gcount.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
   STC_TRY(hipStreamSynchronize(st));
-  for (int k = 0; k < 4; k++) r.t[k] = h.t[k];
+  for (int k = 0; k < 4; k++) {
+    r.t[k] = 0;
+    for (int s = 0; s < kTSlots; s++) r.t[k] += h.t[s][k];
+  }
   (void)classes;
   for (int k = 0; k < 10; k++) {
     r.kern_edges[k] = h.kern_edges[k];
