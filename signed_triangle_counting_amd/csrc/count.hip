@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <mutex>
 #include <vector>
 
@@ -49,18 +50,17 @@ constexpr int kBitmapBlock = 512;   // threads per workgroup of the bitmap kerne
 
 // Device control block, cleared at the start of every counting run.
 // Copies of the bucket histogram that k_bin_hist's workgroups add into (by
-// blockIdx mod kHistSlots), so that no counter takes more than an eighth of
-// the workgroups' atomics; k_bin_table sums them into bucket_cnt.
-constexpr int kHistSlots = 8;
+// blockIdx mod kHistSlots), so that no counter takes more than 1/32 of the
+// workgroups' atomics; k_bin_table sums them into bucket_cnt.
+constexpr int kHistSlots = 32;
 // Copies of the class counters, one 128-byte line each: a wave adds its
 // totals into copy blockIdx mod kTSlots (thousands of waves end together, and
 // one counter pair took all of their atomics); the host sums the copies.
-constexpr int kTSlots = 8;
+constexpr int kTSlots = 64;
 constexpr int kTStride = 16;
 
 struct Ctrl {
   uint32_t bucket_cnt[kBuckets];
-  uint32_t bucket_part[kHistSlots][kBuckets];
   uint32_t len[8];   // merge-lane list, merge-wave list, then vertex lists:
                      // workgroup hash, workgroup bitmap, wave hash, wave bitmap,
                      // workgroup pull, wave pull
@@ -80,6 +80,8 @@ struct Ctrl {
   unsigned long long kern_words[10]; // sum of d+(u) + d+(v) over those edges
   unsigned long long kern_bytes[10]; // bytes each kernel's strategy reads (see k_edge_stats)
   uint8_t table[kBuckets];
+  // last: the host reads the control block up to here
+  uint32_t bucket_part[kHistSlots][kBuckets];
 };
 static_assert(sizeof(Ctrl) % 8 == 0, "ctrl alignment");
 
@@ -2857,7 +2859,7 @@ hipError_t enqueue_batch(const DevCsr &g, const DispatchParams &hp, const Dispat
 
 hipError_t read_count(CountBuffers &cb, bool classes, CountResult &r, hipStream_t st) {
   Ctrl h;
-  STC_TRY(hipMemcpyAsync(&h, cb.gcount.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+  STC_TRY(hipMemcpyAsync(&h, cb.gcount.p, offsetof(Ctrl, bucket_part), hipMemcpyDeviceToHost, st));
   STC_TRY(hipStreamSynchronize(st));
   for (int k = 0; k < 4; k++) {
     r.t[k] = 0;
