@@ -2403,7 +2403,12 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
     uint64_t *const HW = (uint64_t *)(LW + kWaveMaxDeg);
     const uint32_t window = window_words * 16u;
     const uint32_t w1 = nwv, w2 = w1 + nhw, w3 = w2 + nbw, total = w3 + nwp;
-    QueueCursor cur{(blockIdx.x + wave) & 7u, 0u};
+    // A batch's waves start on their own XCD's share (blockIdx mod 8 names the
+    // XCD), so each XCD works on one contiguous range of ranks and its small
+    // graphs stay in that XCD's L2; starting each wave of a workgroup on
+    // another XCD's share cost the 4,096-graph batch 6% (17.89 against
+    // 16.85 ms).  One graph's pops are dealt round-robin over the XCDs.
+    QueueCursor cur{KIND == kKindPullOne ? (blockIdx.x + wave) & 7u : blockIdx.x & 7u, 0u};
     while (true) {
       uint32_t first = 0, count = 0;
       // a batch keeps each XCD on a contiguous range (its graphs stay in one
