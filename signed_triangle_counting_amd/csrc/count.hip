@@ -2046,20 +2046,32 @@ __device__ __forceinline__ void pull_vertex_wave(const DevCsr &g, uint32_t j0, u
   unstage_wave(sw, B);
 }
 
-// Pop one item for the whole workgroup from an XCD queue.
-__device__ __forceinline__ bool block_pop(uint32_t (*ctr)[32], uint32_t len, QueueCursor &cur,
-                                          uint32_t *s_first, uint32_t *s_count,
-                                          uint32_t &first) {
+// Items a workgroup of the two-phase kernel takes per pop in phase 1.
+constexpr uint32_t kBlockPop = 4;
+
+// Pop up to `chunk` items for the whole workgroup from an XCD queue.
+__device__ __forceinline__ bool block_pop_n(uint32_t (*ctr)[32], uint32_t len, QueueCursor &cur,
+                                            uint32_t *s_first, uint32_t *s_count, uint32_t chunk,
+                                            uint32_t &first, uint32_t &count) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t f = 0, c = 0;
-    if (!queue_pop(ctr, len, 1u, cur, f, c)) c = 0;
+    if (!queue_pop(ctr, len, chunk, cur, f, c)) c = 0;
     *s_first = f;
     *s_count = c;
   }
   __syncthreads();
   first = __builtin_amdgcn_readfirstlane(*s_first);
-  return *s_count != 0;
+  count = __builtin_amdgcn_readfirstlane(*s_count);
+  return count != 0;
+}
+
+// Pop one item for the whole workgroup from an XCD queue.
+__device__ __forceinline__ bool block_pop(uint32_t (*ctr)[32], uint32_t len, QueueCursor &cur,
+                                          uint32_t *s_first, uint32_t *s_count,
+                                          uint32_t &first) {
+  uint32_t count;
+  return block_pop_n(ctr, len, cur, s_first, s_count, 1u, first, count);
 }
 
 // One workgroup per source vertex (source lists longer than one wave).
@@ -2358,40 +2370,45 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
     const uint32_t b1 = (nl + lane_item - 1) / lane_item, b2 = b1 + nh, b3 = b2 + nb,
                    total = b3 + np;
     QueueCursor cur{blockIdx.x & 7u, 0u};
-    uint32_t it;
-    while (block_pop(ctrl->queue[q1], total, cur, &s_first, &s_count, it)) {
-      if (ALL && it < b1) {  // merge-lane chunk: kBatchBlock edges per round
-        const uint32_t end = min((it + 1) * lane_item, nl);
-        for (uint32_t i0 = it * lane_item; i0 < end; i0 += kBatchBlock) {
-          const uint32_t i = i0 + tid;
-          const bool valid = i < end;
-          uint32_t c[4] = {0, 0, 0, 0};
-          uint32_t gid = 0;
-          if (valid) {
-            const uint32_t e = lane_list[i];
-            gid = graph_of[g.src[e]];
-            merge_lane_edge<false, false>(g, e, c);  // batches: two-class
+    // kBlockPop consecutive items per pop: a pop is a round trip that every
+    // wave of the workgroup waits for (one item per pop: RMAT-22 5.75 ms,
+    // two 5.62, four 5.61)
+    uint32_t it0, cnt;
+    while (block_pop_n(ctrl->queue[q1], total, cur, &s_first, &s_count, kBlockPop, it0, cnt)) {
+      for (uint32_t it = it0; it < it0 + cnt; it++) {
+        if (ALL && it < b1) {  // merge-lane chunk: kBatchBlock edges per round
+          const uint32_t end = min((it + 1) * lane_item, nl);
+          for (uint32_t i0 = it * lane_item; i0 < end; i0 += kBatchBlock) {
+            const uint32_t i = i0 + tid;
+            const bool valid = i < end;
+            uint32_t c[4] = {0, 0, 0, 0};
+            uint32_t gid = 0;
+            if (valid) {
+              const uint32_t e = lane_list[i];
+              gid = graph_of[g.src[e]];
+              merge_lane_edge<false, false>(g, e, c);  // batches: two-class
+            }
+            lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
           }
-          lane_flush_graph(per_graph, valid, gid, c[0], c[1]);
-        }
-      } else if (ALL && it < b2) {  // hash vertex
-        const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[it - b1]);
-        ga.to(per_graph, graph_of[a]);
-        hash_vertex_block<CLS>(g, a, R, table, tune, acc);
-        for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
-      } else if (ALL && it < b3) {  // bitmap vertex
-        const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[vstride + it - b2]);
-        ga.to(per_graph, graph_of[a]);
-        bitmap_vertex_block<CLS>(g, a, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
+        } else if (ALL && it < b2) {  // hash vertex
+          const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[it - b1]);
+          ga.to(per_graph, graph_of[a]);
+          hash_vertex_block<CLS>(g, a, R, table, tune, acc);
+          for (uint32_t k = tid; k < hash_slots; k += blockDim.x) R[k] = 0u;
+        } else if (ALL && it < b3) {  // bitmap vertex
+          const uint32_t a = __builtin_amdgcn_readfirstlane(vlist[vstride + it - b2]);
+          ga.to(per_graph, graph_of[a]);
+          bitmap_vertex_block<CLS>(g, a, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
+                                   acc);
+          for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
+        } else {  // workgroup-pull chunk
+          const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[4 * vstride + it - b3]);
+          const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[6 * vstride + it - b3]);
+          ga.to(per_graph, graph_of ? graph_of[v] : 0u);
+          pull_vertex_block<CLS>(g, j0, v, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
                                  acc);
-        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
-      } else {  // workgroup-pull chunk
-        const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[4 * vstride + it - b3]);
-        const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[6 * vstride + it - b3]);
-        ga.to(per_graph, graph_of ? graph_of[v] : 0u);
-        pull_vertex_block<CLS>(g, j0, v, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
-                               acc);
-        for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
+          for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
+        }
       }
     }
   }
