@@ -1758,16 +1758,19 @@ struct Staged {
 
 // Workgroup staging (every thread calls it; at most kLowCap low entries).  B
 // must be zero on entry; unstage_block clears the words the list set.
-__device__ __forceinline__ Staged stage_block(const DevCsr &g, uint32_t x, uint32_t *B,
-                                              uint32_t *L, uint32_t *s_nlow, uint32_t window) {
+// The list col[la, ea), whose first and last ranks lo and hi the caller read.
+__device__ __forceinline__ Staged stage_block_at(const DevCsr &g, uint32_t la, uint32_t ea,
+                                                 uint32_t lo, uint32_t hi, uint32_t *B,
+                                                 uint32_t *L, uint32_t *s_nlow,
+                                                 uint32_t window) {
   const uint32_t *__restrict__ col = g.col;
   const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
   Staged st;
-  st.la = __builtin_amdgcn_readfirstlane(g.off[x]);
-  st.ea = __builtin_amdgcn_readfirstlane(g.off[x + 1]);
+  st.la = la;
+  st.ea = ea;
   st.d = st.ea - st.la;
-  st.lo = __builtin_amdgcn_readfirstlane(col[st.la] >> 1);
-  st.hi = __builtin_amdgcn_readfirstlane(col[st.ea - 1] >> 1);
+  st.lo = lo;
+  st.hi = hi;
   st.span = st.hi - st.lo + 1u;
   st.base = window_base(st.lo, st.hi, window);
   for (uint32_t k = tid; k < st.d; k += nthreads) {
@@ -1785,6 +1788,15 @@ __device__ __forceinline__ Staged stage_block(const DevCsr &g, uint32_t x, uint3
   __syncthreads();
   st.nlow = __builtin_amdgcn_readfirstlane(*s_nlow);
   return st;
+}
+
+__device__ __forceinline__ Staged stage_block(const DevCsr &g, uint32_t x, uint32_t *B,
+                                              uint32_t *L, uint32_t *s_nlow, uint32_t window) {
+  const uint32_t la = __builtin_amdgcn_readfirstlane(g.off[x]);
+  const uint32_t ea = __builtin_amdgcn_readfirstlane(g.off[x + 1]);
+  return stage_block_at(g, la, ea, __builtin_amdgcn_readfirstlane(g.col[la] >> 1),
+                        __builtin_amdgcn_readfirstlane(g.col[ea - 1] >> 1), B, L, s_nlow,
+                        window);
 }
 
 __device__ __forceinline__ void unstage_block(const DevCsr &g, const Staged &st, uint32_t *B) {
@@ -1921,19 +1933,6 @@ __device__ __forceinline__ void bitmap_vertex_wave(const DevCsr &g, uint32_t a, 
 // so staging need not wait for a load of the chunk's first edge.
 // An in-edge is pull iff its bucket (|N+(u) after v|, d+(v)) says pull (v was
 // listed only if its list fits).
-struct PullChunk {
-  uint32_t v, t0, t1;
-};
-
-__device__ __forceinline__ PullChunk pull_chunk(const DevCsr &g, uint32_t j0, uint32_t v,
-                                                uint32_t chunk) {
-  PullChunk c;
-  c.t0 = j0;
-  c.v = v;
-  c.t1 = min(j0 + chunk, __builtin_amdgcn_readfirstlane(g.tin_off[v + 1]));
-  return c;
-}
-
 // Pull streaming.  PACKED: the lists are streamed packed (H: the wave's
 // kPackWords-word records and start masks), which pays off for the short lists of the
 // wave kernel's targets; the workgroup kernel's targets stream lists long
@@ -1955,18 +1954,28 @@ __device__ __forceinline__ void pull_stream(const DevCsr &g, uint32_t first, uin
     stream_rows<CLS>(g, first, t1, step, PullMeta{g}, keep, probe, acc);
 }
 
-// The chunk of target v's in-edges starting at position j0.
+// The header of a workgroup-pull chunk: its in-edges [j0, t1) of target v,
+// whose list is col[la, ea) with first and last ranks lo and hi, and v's
+// graph.  The two-phase kernel gathers the headers of a pop's items in one
+// pass (three rounds of loads for up to kBlockPop items), so an item starts
+// staging at once instead of after a chain of four dependent loads.
+constexpr uint32_t kHdrFields = 8;
+enum { kHdrJ0, kHdrV, kHdrLa, kHdrEa, kHdrT1, kHdrLo, kHdrHi, kHdrGraph };
+
 template <bool CLS>
-__device__ __forceinline__ void pull_vertex_block(const DevCsr &g, uint32_t j0, uint32_t v,
-                                                  uint32_t *B, uint32_t *L, uint32_t *s_nlow,
-                                                  const uint8_t *table, const TuneParams &tune,
-                                                  Acc<CLS> &acc) {
+__device__ __forceinline__ void pull_vertex_block_at(const DevCsr &g, const uint32_t *hdr,
+                                                     uint32_t *B, uint32_t *L, uint32_t *s_nlow,
+                                                     const uint8_t *table,
+                                                     const TuneParams &tune, Acc<CLS> &acc) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nwaves = blockDim.x >> 6;
-  const PullChunk c = pull_chunk(g, j0, v, 1u << tune.pull_block_chunk_log2);
-  const Staged st = stage_block(g, c.v, B, L, s_nlow, 1u << tune.bitmap_span_log2);
+  auto f = [&](int k) { return __builtin_amdgcn_readfirstlane(hdr[k]); };
+  const uint32_t j0 = f(kHdrJ0);
+  const uint32_t t1 = min(j0 + (1u << tune.pull_block_chunk_log2), f(kHdrT1));
+  const Staged st = stage_block_at(g, f(kHdrLa), f(kHdrEa), f(kHdrLo), f(kHdrHi), B, L, s_nlow,
+                                   1u << tune.bitmap_span_log2);
   st.with_probe(B, L, [&](const auto &probe) {
-    pull_stream<CLS, false>(g, c.t0 + wave, c.t1, nwaves, st.d, table, probe, nullptr, acc);
+    pull_stream<CLS, false>(g, j0 + wave, t1, nwaves, st.d, table, probe, nullptr, acc);
   });
   unstage_block(g, st, B);
 }
@@ -2373,8 +2382,28 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
     // kBlockPop consecutive items per pop: a pop is a round trip that every
     // wave of the workgroup waits for (one item per pop: RMAT-22 5.75 ms,
     // two 5.62, four 5.61)
+    // the pop's pull-chunk headers: the last words of the region, past every
+    // phase-1 structure (phase 2 reuses them once phase 1 is over)
+    uint32_t *const Hdr = R + region_words - kBlockPop * kHdrFields;
     uint32_t it0, cnt;
     while (block_pop_n(ctrl->queue[q1], total, cur, &s_first, &s_count, kBlockPop, it0, cnt)) {
+      if (it0 + cnt > b3) {  // the pop holds pull chunks: their headers, thread k item k
+        if (tid < cnt && it0 + tid >= b3) {
+          const uint32_t k = it0 + tid - b3;
+          uint32_t *h = Hdr + tid * kHdrFields;
+          const uint32_t j0 = vlist[4 * vstride + k], v = vlist[6 * vstride + k];
+          const uint32_t la = g.off[v], ea = g.off[v + 1];
+          h[kHdrJ0] = j0;
+          h[kHdrV] = v;
+          h[kHdrLa] = la;
+          h[kHdrEa] = ea;
+          h[kHdrT1] = g.tin_off[v + 1];
+          h[kHdrGraph] = graph_of ? graph_of[v] : 0u;
+          h[kHdrLo] = g.col[la] >> 1;
+          h[kHdrHi] = g.col[ea - 1] >> 1;
+        }
+        __syncthreads();
+      }
       for (uint32_t it = it0; it < it0 + cnt; it++) {
         if (ALL && it < b1) {  // merge-lane chunk: kBatchBlock edges per round
           const uint32_t end = min((it + 1) * lane_item, nl);
@@ -2402,11 +2431,10 @@ __global__ __launch_bounds__(kBatchBlock, 8) void k_two_phase(
                                    acc);
           for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
         } else {  // workgroup-pull chunk
-          const uint32_t j0 = __builtin_amdgcn_readfirstlane(vlist[4 * vstride + it - b3]);
-          const uint32_t v = __builtin_amdgcn_readfirstlane(vlist[6 * vstride + it - b3]);
-          ga.to(per_graph, graph_of ? graph_of[v] : 0u);
-          pull_vertex_block<CLS>(g, j0, v, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
-                                 acc);
+          const uint32_t *h = Hdr + (it - it0) * kHdrFields;
+          ga.to(per_graph, __builtin_amdgcn_readfirstlane(h[kHdrGraph]));
+          pull_vertex_block_at<CLS>(g, h, R, R + bitmap_words + kGuard, &s_nlow, table, tune,
+                                    acc);
           for (uint32_t k = tid; k < kLowCap; k += blockDim.x) R[bitmap_words + kGuard + k] = 0u;
         }
       }
@@ -2821,11 +2849,13 @@ hipError_t resident_blocks(int *out, const void *kern, int threads, size_t lds) 
 static size_t batch_region_words(const TuneParams &t) {
   const size_t bitmap = (size_t)((1u << t.bitmap_span_log2) / 16u) + kGuard + kLowCap;
   const size_t hash = (size_t)4u << t.hash_max_log2;
+  // phase 1: the bitmap or hash table, then the pop's pull-chunk headers
+  const size_t phase1 = std::max(bitmap, hash) + kBlockPop * kHdrFields;
   // phase 2: one slot per wave (hash table or window + low array, then heads)
   const size_t wave_slot = std::max<size_t>(kWaveCap, (1u << t.wave_window_log2) / 16u) +
                            kGuard + kWaveMaxDeg + kPackWords;
   const size_t waves = (size_t)kWavesPerBatchBlock * wave_slot;
-  return std::max(bitmap, std::max(hash, waves));
+  return std::max(phase1, waves);
 }
 
 size_t batch_lds_bytes(const TuneParams &t) { return 4u * batch_region_words(t); }
